@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Band-of-sight throughput benchmark (one MI355X).
+
+One step = one full total-viewshed pass over the workload: all 180 sectors,
+forward and backward bands of every computable point, per-point cumulative
+surfaces.  The DEM (int16) and the 180 sectors' band data are resident in
+HBM before timing starts; band data computation (device sort + compile) is
+timed separately and reported as `bands_ms`.
+
+Prints one JSON line.  `value` is point-steps/s of the batched method
+(int16 heights); the plain one-sector-per-launch FP32 method is measured in
+the same run and reported under `plain`, with the TVS relative error between
+the two.  `cpu_baseline` times the oracle's literal restatement of the
+reference CPU loop on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "total-viewsheds_amd"))
+
+METRIC = ("band-of-sight point-steps/s on 1201² DEM; batched vs plain at "
+          "≤1e-4 TVS rel err")
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def workload(name: str):
+    from tvs_amd import demgen
+    return {"main-1201": demgen.main_1201, "small-201": demgen.small_201,
+            "large-3601": demgen.large_3601}[name]()
+
+
+def time_method(e, method: str, steps: int, warmup: int):
+    for _ in range(warmup):
+        e.run(0, 180, method)
+    e.sync()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e.run(0, 180, method)
+        kernel_ms.append(e.timing().walk_ms)   # waits for this step's events
+    e.sync()
+    wall = time.perf_counter() - t0
+    return wall * 1000.0 / steps, float(np.mean(kernel_ms)), e.timing()
+
+
+def cpu_baseline(wl, sectors: int, threads: int):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    dem = O.dem_new(wl.width, wl.spacing, wl.max_line_of_sight)
+    picked = list(range(0, 180, max(1, 180 // sectors)))[:sectors]
+    deltas, dists = O.band_data_all(dem, picked)
+    elev = wl.elevations.astype(np.float32)
+    t0 = time.perf_counter()
+    O.walk_sectors(dem, elev, wl.observer_height, deltas, dists, n_threads=threads)
+    dt = time.perf_counter() - t0
+    steps = len(picked) * 2 * dem.computable_points_count * dem.n_deltas
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": steps / dt, "unit": "point-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": (f"{len(picked)} of 180 sectors (every {180 // len(picked)}th) of "
+                       f"{wl.name}, walk only, oracle C restatement of the reference "
+                       f"CPU loop, {threads} thread(s), {dt:.1f} s on {cpu}")}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__,
+                                 formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="main-1201",
+                    choices=["main-1201", "small-201", "large-3601"])
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain method")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sectors", type=int, default=18)
+    ap.add_argument("--cpu-threads", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the engine runs on a single MI355X "
+              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        return 2
+
+    import tvs_amd
+    wl = workload(args.workload)
+    log(f"workload {wl.name}: {wl.width}x{wl.width} @ {wl.spacing} m, "
+        f"los {wl.max_line_of_sight} m")
+    e = tvs_amd.Engine(wl.width, wl.spacing, wl.max_line_of_sight, wl.rings_per_km,
+                       wl.observer_height, 0)
+    g = e.geometry
+    e.upload(wl.elevations)
+    t0 = time.perf_counter()
+    e.compute_bands(0, 180)
+    bands_wall_ms = (time.perf_counter() - t0) * 1000.0
+    bands_ms = e.timing().bands_ms
+    log(f"band data for 180 sectors: {bands_ms:.1f} ms device, {bands_wall_ms:.1f} ms wall")
+    point_steps = 180 * 2 * g.computable_points_count * g.band_deltas_size
+
+    plain = None
+    if not args.no_plain:
+        log("timing plain (FP32 heights, one sector per launch)")
+        p_ms, p_kernel_ms, _ = time_method(e, "plain", args.steps, args.warmup)
+        plain = {"value": point_steps / (p_ms / 1000.0), "ms_per_step": p_ms,
+                 "kernel_ms": p_kernel_ms, "launches_per_step": 180,
+                 "achieved_GBps": point_steps * 4 / (p_kernel_ms / 1000.0) / 1e9}
+        log(f"plain: {p_ms:.2f} ms/step, {plain['value']:.3e} point-steps/s")
+
+    log("timing batched (int16 heights, all sectors in one launch)")
+    b_ms, b_kernel_ms, tm = time_method(e, "batched", args.steps, args.warmup)
+    value = point_steps / (b_ms / 1000.0)
+    log(f"batched: {b_ms:.2f} ms/step (walk kernel {b_kernel_ms:.2f} ms, reduce "
+        f"{tm.reduce_ms:.2f} ms), {value:.3e} point-steps/s")
+
+    # Correctness of the timed configuration: one fresh pass of each method.
+    e.reset_surfaces()
+    e.run(0, 180, "batched")
+    tvs_b = e.surfaces()
+    tvs_b64 = e.surfaces_f64()
+    e.reset_surfaces()
+    e.run(0, 180, "plain")
+    tvs_p = e.surfaces()
+    e.close()
+    rel_err = float(np.max(np.abs(tvs_b.astype(np.float64) - tvs_p) / tvs_p))
+    rel_err64 = float(np.max(np.abs(tvs_b64 - tvs_p) / tvs_b64))
+    bit_exact = bool(np.array_equal(tvs_b.view(np.uint32), tvs_p.view(np.uint32)))
+
+    bytes_per_launch = point_steps * 2          # one int16 height per point-step
+    achieved = bytes_per_launch / (b_kernel_ms / 1000.0) / 1e9
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "walk_batched_main-1201_traffic.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "point-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": b_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": wl.name,
+            "dem": f"{wl.width}x{wl.width} int16, {wl.spacing:g} m spacing, seeded "
+                   "spectral fractal" + (" + ridges" if wl.name != "small-201" else ""),
+            "max_line_of_sight_m": wl.max_line_of_sight,
+            "observer_height_m": wl.observer_height,
+            "rings_per_km": wl.rings_per_km,
+            "sectors": 180,
+            "method": "batched",
+            "heights": "int16",
+            "point_steps_per_step": point_steps,
+            "steps_per_band": g.band_deltas_size,
+            "real_steps_per_band": g.real_steps,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": traffic,
+            "kernel": "walk_batched_kernel<int16_t, fast divide>",
+            "kernel_ms": b_kernel_ms,
+            "bytes_per_launch": bytes_per_launch,
+        },
+        "plain": plain,
+        "tvs_rel_err_batched_vs_plain": rel_err,
+        "tvs_bit_exact_batched_vs_plain": bit_exact,
+        "tvs_rel_err_f64_sum_vs_plain": rel_err64,
+        "reduce_ms": tm.reduce_ms,
+        "bands_ms": bands_ms,
+    }
+    if not args.no_cpu_baseline:
+        log(f"CPU baseline: oracle walk over {args.cpu_sectors} sectors, "
+            f"{args.cpu_threads} thread(s)")
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_sectors, args.cpu_threads)
+        log(f"CPU: {out['cpu_baseline']['value']:.3e} point-steps/s")
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

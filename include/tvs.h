@@ -1,0 +1,220 @@
+/*
+ * tvs.h -- C ABI of the MI355X total-viewshed engine (libtvs_hip.so).
+ *
+ * Plain C types only: pointers, sizes, scalars.  Every entry point returns a
+ * tvs_status and never aborts.  One context is used by one host thread at a
+ * time; each context owns one HIP stream and its device buffers.
+ *
+ * Each declaration names the reference interface it replaces
+ * (tombh/total-viewsheds; paths relative to crates/total-viewsheds/src/
+ * unless written crates/shader/src/...).
+ */
+#ifndef TVS_H
+#define TVS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TVS_ABI_VERSION 1u
+
+/* Status codes.  The non-zero values map the reference's `bail!` and panic
+ * sites onto error returns. */
+typedef enum tvs_status {
+    TVS_OK = 0,
+    TVS_ERR_INVALID_ARGUMENT = 1, /* null pointer, zero width, bad enum       */
+    TVS_ERR_LOS_TOO_LONG = 2,     /* dem.rs:56-62 bail!                        */
+    TVS_ERR_BAND_TOO_SHORT = 3,   /* band_of_sight.rs:158-168 index panic
+                                     (band_size <= 2) or a window past the DEM */
+    TVS_ERR_BAND_OUT_OF_DEM = 4,  /* kernel.rs:164 index panic: a band leaves
+                                     the elevation array                       */
+    TVS_ERR_SIZE = 5,             /* a buffer length does not match the DEM    */
+    TVS_ERR_STATE = 6,            /* call order (e.g. run before upload)       */
+    TVS_ERR_NO_DEVICE = 7,        /* no HIP device / gpu.rs:84-89 analogue     */
+    TVS_ERR_OUT_OF_MEMORY = 8,
+    TVS_ERR_HIP = 9,              /* any other HIP runtime error               */
+    TVS_ERR_IO = 10               /* file read/write (.bt, band cache)         */
+} tvs_status;
+
+/* Walk implementations (the `ComputeType` slot of config.rs:57-64 /
+ * compute.rs:188-198 that `Cuda => unimplemented!()` leaves empty). */
+typedef enum tvs_method {
+    /* One sector per launch, FP32 heights, the reference loop literally
+     * (IEEE f32 divide per step), one thread per band. */
+    TVS_METHOD_PLAIN = 0,
+    /* All requested sectors in one launch: per-step constants hoisted, the
+     * divide done as a correctly rounded reciprocal-plus-residual sequence,
+     * heights read as int16 when an int16 DEM was uploaded, per-sector band
+     * surfaces reduced in sector order (bit-identical f32 TVS) plus an FP64
+     * per-point sum. */
+    TVS_METHOD_BATCHED = 1
+} tvs_method;
+
+/* dem.rs:11-84 (`DEM` scalars) + band_of_sight.rs:54-58, 95, 149-150. */
+typedef struct tvs_geometry {
+    uint32_t dem_width;               /* w                                  */
+    uint32_t dem_size;                /* w*w                                */
+    float scale;                      /* metres per point                   */
+    uint32_t max_line_of_sight;       /* metres                             */
+    uint32_t max_los_as_points;       /* L  (dem.rs:53)                     */
+    uint32_t computable_points_count; /* N  (dem.rs:87-94)                  */
+    uint32_t tvs_width;               /* T = isqrt(N) (dem.rs:82)           */
+    uint32_t band_size;               /* B = L + 1 (dem.rs:78)              */
+    uint32_t band_deltas_size;        /* B - 1: steps per band              */
+    uint32_t band_samples;            /* floor(2B*sqrt2) (band_of_sight:95) */
+    uint32_t real_steps;              /* min(2B - bs/2, B-1): filled prefix */
+} tvs_geometry;
+
+/* Context parameters: the arguments of `Compute::new` (compute.rs:27-33)
+ * and of `GPU::new` (gpu.rs:55-61) that are not buffers. */
+typedef struct tvs_params {
+    uint32_t dem_width;
+    float scale;
+    uint32_t max_line_of_sight;  /* metres; 0 = default (main.rs:36-38)      */
+    float rings_per_km;          /* compute.rs:94-98 (reference default 5)  */
+    float observer_height;       /* metres (config.rs:30-36; reference 1.8
+                                    in Compute::new, compute.rs:41)          */
+    uint32_t ring_capture_sectors; /* sectors whose ring records one run
+                                      keeps (0 = none, as the SPIR-V build,
+                                      kernel.rs:302-309)                    */
+} tvs_params;
+
+/* Per-phase device times of the last run / band computation, from HIP
+ * events on the context's stream (milliseconds). */
+typedef struct tvs_timing {
+    float walk_ms;         /* all walk-kernel launches of the last run      */
+    float reduce_ms;       /* sector-order reduction (batched method)       */
+    float bands_ms;        /* last tvs_compute_bands: keys, sorts, compile  */
+    uint32_t walk_launches;
+    uint64_t point_steps;  /* sectors * 2N * (B-1) walked by the last run   */
+} tvs_timing;
+
+typedef struct tvs_ctx tvs_ctx;
+
+/* ---- geometry (host only, no device needed) --------------------------- */
+
+/* `DEM::new` (dem.rs:44-84).  Returns TVS_ERR_LOS_TOO_LONG as dem.rs:56-62
+ * bails. */
+tvs_status tvs_dem_geometry(uint32_t dem_width, float scale,
+                            uint32_t max_line_of_sight, tvs_geometry *out);
+
+/* Default max line of sight `(w div 3) * scale` (main.rs:36-38). */
+uint32_t tvs_default_max_line_of_sight(uint32_t dem_width, double scale);
+
+/* `Compute::ring_count_per_band` (compute.rs:94-98). */
+uint32_t tvs_ring_count_per_band(float rings_per_km,
+                                 uint32_t max_line_of_sight);
+
+/* ---- context ------------------------------------------------------------ */
+
+/* `GPU::new` (gpu.rs:55-176) + `Compute::new` (compute.rs:27-84): device,
+ * stream and the per-DEM constants (kernel.rs:38-58).  Elevations come
+ * separately (tvs_upload_dem_*). */
+tvs_status tvs_create(const tvs_params *params, tvs_ctx **out);
+
+/* Frees every device buffer and the stream. */
+tvs_status tvs_destroy(tvs_ctx *ctx);
+
+/* Geometry of the context's DEM (as tvs_dem_geometry). */
+tvs_status tvs_get_geometry(const tvs_ctx *ctx, tvs_geometry *out);
+
+/* Elevation upload, `GPU::setup_buffers` elevations (gpu.rs:195-200), from
+ * the f32 array main.rs:51-56 builds.  Copies; the caller keeps ownership. */
+tvs_status tvs_upload_dem_f32(tvs_ctx *ctx, const float *elevations,
+                              size_t count);
+
+/* Elevation upload from the `.bt` int16 samples (input.rs:135-171) without
+ * the f32 widening.  Also keeps an f32 copy for TVS_METHOD_PLAIN; the
+ * widening is exact (main.rs:52-53). */
+tvs_status tvs_upload_dem_i16(tvs_ctx *ctx, const int16_t *elevations,
+                              size_t count);
+
+/* ---- band data ---------------------------------------------------------- */
+
+/* `DEM::calculate_axes` + `DEM::compile_band_data` (dem.rs:135-139,
+ * axes.rs:103-130, band_of_sight.rs:61-173) for sectors
+ * [first, first+count), on the device: f64 rotated-coordinate keys, stable
+ * LSD radix sorts, inverse sight map, band compile.  The band data stays
+ * resident for the walk.  Synchronous. */
+tvs_status tvs_compute_bands(tvs_ctx *ctx, uint32_t first_sector,
+                             uint32_t sector_count);
+
+/* Band data supplied from outside (a cache file, another producer): the
+ * inputs of `GPU::run` (gpu.rs:387-394), `n` = band_deltas_size. */
+tvs_status tvs_set_band(tvs_ctx *ctx, uint32_t sector, const int32_t *deltas,
+                        const float *distances, size_t n);
+
+/* Copy a sector's resident band data back (the `dem.band_deltas` /
+ * `dem.band_distances` pair of dem.rs:15-17). */
+tvs_status tvs_get_band(tvs_ctx *ctx, uint32_t sector, int32_t *deltas,
+                        float *distances, size_t n);
+
+/* Run the device `Axes::compute` (axes.rs:103-130) for one sector and copy
+ * its outputs back: `distances` (axes.distances, f32[w*w]),
+ * `sight_ordered_map` (u32[w*w]), `sector_ordered` (u32[w*w]).  Any output
+ * pointer may be NULL. */
+tvs_status tvs_compute_axes(tvs_ctx *ctx, uint32_t sector, float *distances,
+                            uint32_t *sight_ordered_map,
+                            uint32_t *sector_ordered);
+
+/* ---- walk ---------------------------------------------------------------- */
+
+/* `Compute::run`'s sector loop (compute.rs:101-115) over resident band data
+ * for sectors [first, first+count), with `compute_sector` dispatching to the
+ * chosen method (compute.rs:181-203).  Surfaces accumulate across calls
+ * like `cumulative_surfaces` (compute.rs:102, kernel.rs:254) until
+ * tvs_reset_surfaces.  Ring records of the first `ring_capture_sectors`
+ * sectors of this call are kept (test-mode layout, compute.rs:46-47).
+ * Asynchronous on the context's stream: read results or tvs_sync. */
+tvs_status tvs_run_sectors(tvs_ctx *ctx, uint32_t first_sector,
+                           uint32_t sector_count, tvs_method method);
+
+/* One sector from host band data, returning the cumulative surfaces:
+ * exactly `GPU::run` (gpu.rs:387-468) as called by compute_sector_vulkan
+ * (compute.rs:217-225).  `cumulative_out` may be NULL.  Synchronous. */
+tvs_status tvs_run_sector_with_band(tvs_ctx *ctx, const int32_t *deltas,
+                                    const float *distances, size_t n,
+                                    float *cumulative_out, size_t n_out);
+
+/* Zero the cumulative surfaces (compute.rs:102). */
+tvs_status tvs_reset_surfaces(tvs_ctx *ctx);
+
+/* Cumulative per-point surfaces, f32[N] (the first element of
+ * `Compute::run`'s return, compute.rs:114).  Synchronous. */
+tvs_status tvs_read_surfaces(tvs_ctx *ctx, float *out, size_t n);
+
+/* The same sums carried in double across sectors (TVS_METHOD_BATCHED keeps
+ * them; PLAIN leaves them untouched).  Synchronous. */
+tvs_status tvs_read_surfaces_f64(tvs_ctx *ctx, double *out, size_t n);
+
+/* Ring records of captured slot `slot` (the sector first_sector+slot of the
+ * last run): 2N*R u32 in the layout kernel.rs:141-243 writes (count, PoV
+ * id, then alternating close/open DEM ids), zero elsewhere.  Writes past a
+ * band's R slots are dropped and counted instead of spilling into the next
+ * band (kernel.rs:141).  Synchronous. */
+tvs_status tvs_read_rings(tvs_ctx *ctx, uint32_t slot, uint32_t *out,
+                          size_t n);
+
+/* Ring events dropped by the last run because a band had more than R-1. */
+tvs_status tvs_ring_overflow_count(tvs_ctx *ctx, uint64_t *out);
+
+/* Wait for the context's stream. */
+tvs_status tvs_sync(tvs_ctx *ctx);
+
+/* Device times of the last run and band computation. */
+tvs_status tvs_get_timing(tvs_ctx *ctx, tvs_timing *out);
+
+/* Message for the last failing call on `ctx` (or on a NULL context). */
+const char *tvs_last_error(const tvs_ctx *ctx);
+
+/* Static name of a status code. */
+const char *tvs_status_string(tvs_status status);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TVS_H */

@@ -1,0 +1,259 @@
+"""HIP engine vs the oracle on seeded synthetic DEMs, through the C ABI.
+
+Bit-exact: band data (sort order, deltas, distances), full Axes orders,
+cumulative f32 surfaces of both walk methods, ring records.  At the
+1201 x 1201 size the oracle runs on a sample of sectors; the full 180-sector
+run is checked by size-independent properties (the two methods agree bit
+for bit; the f64 sums agree with the f32 ones within 1e-5 relative)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import tvs_amd
+from tvs_amd import demgen, _lib
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+# --- 201 x 201, all 180 sectors -------------------------------------------
+
+@pytest.fixture(scope="module", params=[0, 1])
+def small(request, oracle):
+    wl = demgen.small_201(seed=request.param)
+    note(f"small-201 seed {request.param}: oracle band data + walk")
+    dem = oracle.dem_new(wl.width, wl.spacing, wl.max_line_of_sight)
+    deltas, dists = oracle.band_data_all(dem, range(180))
+    elev = wl.elevations.astype(np.float32)
+    cum, _ = oracle.walk_sectors(dem, elev, wl.observer_height, deltas, dists,
+                                 n_threads=THREADS)
+    return wl, dem, deltas, dists, cum
+
+
+def make_engine(wl, capture=0, rings_per_km=None):
+    return tvs_amd.Engine(wl.width, wl.spacing, wl.max_line_of_sight,
+                          wl.rings_per_km if rings_per_km is None else rings_per_km,
+                          wl.observer_height, capture)
+
+
+def test_small_band_data_bit_exact(small):
+    wl, dem, deltas, dists, _ = small
+    note("small-201: device sort + band compile, 180 sectors")
+    with make_engine(wl) as e:
+        e.compute_bands(0, 180)
+        for s in range(180):
+            d, x = e.get_band(s)
+            np.testing.assert_array_equal(d, deltas[s], err_msg=f"deltas sector {s}")
+            np.testing.assert_array_equal(x.view(np.uint32), dists[s].view(np.uint32),
+                                          err_msg=f"distances sector {s}")
+
+
+@pytest.mark.parametrize("sector", [0, 1, 33, 45, 46, 90, 134, 135, 179])
+def test_small_axes_bit_exact(oracle, sector):
+    note(f"axes 201 sector {sector}")
+    with tvs_amd.Engine(201, 100.0, 6700) as e:
+        dist, sight_map, order = e.compute_axes(sector)
+    od, osm, oso = oracle.axes_compute(201, sector)
+    np.testing.assert_array_equal(dist.view(np.uint32), od.view(np.uint32))
+    np.testing.assert_array_equal(sight_map, osm)
+    np.testing.assert_array_equal(order, oso)
+
+
+@pytest.mark.parametrize("method", ["plain", "batched"])
+@pytest.mark.parametrize("dtype", [np.int16, np.float32])
+def test_small_tvs_bit_exact(small, method, dtype):
+    wl, dem, _, _, cum = small
+    note(f"small-201 TVS {method} {np.dtype(dtype).name}")
+    with make_engine(wl) as e:
+        e.upload(wl.elevations.astype(dtype))
+        e.compute_bands(0, 180)
+        e.run(0, 180, method)
+        got = e.surfaces()
+        got64 = e.surfaces_f64()
+    np.testing.assert_array_equal(got.view(np.uint32), cum.view(np.uint32))
+    if method == "batched":
+        assert np.max(np.abs(got64 - cum) / np.maximum(got64, 1e-30)) < 1e-5
+
+
+def test_small_tvs_from_host_band_data(small):
+    """tvs_set_band (band data from outside) feeds the same walk."""
+    wl, dem, deltas, dists, cum = small
+    with make_engine(wl) as e:
+        e.upload(wl.elevations)
+        for s in range(180):
+            e.set_band(s, deltas[s], dists[s])
+        e.run(0, 180, "batched")
+        np.testing.assert_array_equal(e.surfaces(), cum)
+
+
+def test_small_gpu_run_mirror(small, oracle):
+    """tvs_run_sector_with_band is GPU::run (gpu.rs:387-468): one sector of
+    host band data in, cumulative surfaces out."""
+    wl, dem, deltas, dists, _ = small
+    elev = wl.elevations.astype(np.float32)
+    want = np.zeros(dem.computable_points_count, np.float32)
+    with make_engine(wl) as e:
+        e.upload(elev)
+        for s in (0, 17, 91):
+            got = e.run_sector_with_band(deltas[s], dists[s])
+            oracle.walk_sectors(dem, elev, wl.observer_height, deltas[s:s + 1],
+                                dists[s:s + 1], cumulative=want)
+            np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("method", ["plain", "batched"])
+def test_small_rings_bit_exact(small, oracle, method):
+    wl, dem, deltas, dists, _ = small
+    first, count = 40, 4
+    note(f"small-201 rings {method}")
+    R = oracle.ring_count_per_band(wl.rings_per_km, wl.max_line_of_sight)
+    _, want = oracle.walk_sectors(dem, wl.elevations.astype(np.float32), wl.observer_height,
+                                  deltas[first:first + count], dists[first:first + count],
+                                  rings_per_band=R, rings=True)
+    with make_engine(wl, capture=count) as e:
+        e.upload(wl.elevations)
+        e.compute_bands(first, count)
+        e.run(first, count, method)
+        for r in range(count):
+            np.testing.assert_array_equal(e.rings(r), want[r], err_msg=f"slot {r}")
+        assert e.ring_overflow() == 0
+
+
+# --- 1201 x 1201 -------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def main(oracle):
+    wl = demgen.main_1201(seed=0)
+    note("main-1201: engine bands for 180 sectors")
+    e = make_engine(wl)
+    e.upload(wl.elevations)
+    e.compute_bands(0, 180)
+    dem = oracle.dem_new(wl.width, wl.spacing, wl.max_line_of_sight)
+    yield wl, dem, e
+    e.close()
+
+
+SAMPLE = [0, 1, 2, 44, 45, 46, 89, 90, 91, 134, 135, 136, 178, 179]
+
+
+def test_main_band_data_bit_exact(main, oracle):
+    wl, dem, e = main
+    for s in SAMPLE:
+        note(f"main-1201 band sector {s}")
+        want_d, want_x = oracle.band_data(dem, s)
+        d, x = e.get_band(s)
+        np.testing.assert_array_equal(d, want_d, err_msg=f"deltas sector {s}")
+        np.testing.assert_array_equal(x.view(np.uint32), want_x.view(np.uint32))
+
+
+@pytest.mark.parametrize("sector", [0, 45, 100])
+def test_main_axes_bit_exact(main, oracle, sector):
+    wl, dem, e = main
+    note(f"main-1201 axes sector {sector}")
+    dist, sight_map, order = e.compute_axes(sector)
+    od, osm, oso = oracle.axes_compute(wl.width, sector)
+    np.testing.assert_array_equal(dist.view(np.uint32), od.view(np.uint32))
+    np.testing.assert_array_equal(sight_map, osm)
+    np.testing.assert_array_equal(order, oso)
+
+
+@pytest.mark.parametrize("method", ["plain", "batched"])
+def test_main_tvs_sample_bit_exact(main, oracle, method):
+    wl, dem, e = main
+    sectors = [0, 45, 91]
+    note(f"main-1201 TVS sample {method}")
+    elev = wl.elevations.astype(np.float32)
+    want = np.zeros(dem.computable_points_count, np.float32)
+    e.reset_surfaces()
+    for s in sectors:
+        d, x = oracle.band_data(dem, s)
+        oracle.walk_sectors(dem, elev, wl.observer_height, d[None], x[None],
+                            n_threads=THREADS, cumulative=want)
+        e.run(s, 1, method)
+    np.testing.assert_array_equal(e.surfaces().view(np.uint32), want.view(np.uint32))
+
+
+def test_main_full_run_methods_agree(main):
+    """All 180 sectors: batched int16 == plain f32 bit for bit, f64 sums
+    within 1e-5 of the f32 ones, every surface positive."""
+    wl, dem, e = main
+    note("main-1201 full plain")
+    e.reset_surfaces()
+    e.run(0, 180, "plain")
+    plain = e.surfaces()
+    note("main-1201 full batched")
+    e.reset_surfaces()
+    e.run(0, 180, "batched")
+    batched = e.surfaces()
+    b64 = e.surfaces_f64()
+    np.testing.assert_array_equal(batched.view(np.uint32), plain.view(np.uint32))
+    assert np.all(batched > 0)
+    assert np.max(np.abs(b64 - batched) / b64) < 1e-5
+
+
+def test_main_rings_one_sector(main, oracle):
+    wl, dem, _ = main
+    s = 46
+    note("main-1201 rings sector 46")
+    R = oracle.ring_count_per_band(wl.rings_per_km, wl.max_line_of_sight)
+    d, x = oracle.band_data(dem, s)
+    _, want = oracle.walk_sectors(dem, wl.elevations.astype(np.float32), wl.observer_height,
+                                  d[None], x[None], rings_per_band=R, rings=True)
+    with make_engine(wl, capture=1) as e:
+        e.upload(wl.elevations)
+        e.set_band(s, d, x)
+        e.run(s, 1, "batched")
+        np.testing.assert_array_equal(e.rings(0), want[0])
+
+
+# --- error paths through the ABI --------------------------------------------
+
+def test_errors_are_status_codes():
+    with pytest.raises(tvs_amd.TvsError) as err:
+        tvs_amd.Engine(9, 1.0, 5)
+    assert err.value.status == _lib.ERR_LOS_TOO_LONG
+    with tvs_amd.Engine(9, 1.0, 3) as e:
+        with pytest.raises(tvs_amd.TvsError) as err:
+            e.run(0, 1)
+        assert err.value.status == _lib.ERR_STATE          # no elevations
+        with pytest.raises(tvs_amd.TvsError) as err:
+            e.upload(np.zeros(80, np.int16))
+        assert err.value.status == _lib.ERR_SIZE
+        e.upload(np.zeros(81, np.int16))
+        with pytest.raises(tvs_amd.TvsError) as err:
+            e.run(0, 1)
+        assert err.value.status == _lib.ERR_STATE          # no band data
+        with pytest.raises(tvs_amd.TvsError) as err:
+            e.run(170, 20)
+        assert err.value.status == _lib.ERR_INVALID_ARGUMENT
+        with pytest.raises(tvs_amd.TvsError) as err:
+            e.set_band(0, np.array([1000, 0, 0]), np.array([1.0, 0, 0], np.float32))
+        assert err.value.status == _lib.ERR_BAND_OUT_OF_DEM
+    with tvs_amd.Engine(9, 1.0, 1) as e:                   # band_size 2
+        with pytest.raises(tvs_amd.TvsError) as err:
+            e.compute_bands(0, 1)
+        assert err.value.status == _lib.ERR_BAND_TOO_SHORT
+
+
+def test_ring_overflow_is_counted_not_spilled(small):
+    """With too few reserved rings the reference writes into the next band
+    (kernel.rs:141); the engine drops and counts instead, and the surfaces
+    are unaffected."""
+    wl, dem, _, _, cum = small
+    with make_engine(wl, capture=180, rings_per_km=0.5) as e:
+        assert e.rings_per_band == 3
+        e.upload(wl.elevations)
+        e.compute_bands(0, 180)
+        e.run(0, 180, "batched")
+        assert e.ring_overflow() > 0
+        np.testing.assert_array_equal(e.surfaces(), cum)
+        r = e.rings(0).reshape(-1, 3)
+        assert np.all(r[:, 0] % 2 == 0)

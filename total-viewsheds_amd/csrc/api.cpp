@@ -1,0 +1,577 @@
+// api.cpp -- the extern "C" boundary (include/tvs.h): context lifetime,
+// uploads, band data, walk dispatch, readback, errors.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "engine.hpp"
+
+namespace tvs {
+uint32_t max_band_samples();
+}
+
+namespace {
+
+thread_local std::string g_error;  // errors with no context to hold them
+
+tvs_status fail(tvs_ctx *c, tvs_status st, const std::string &msg) {
+    (c ? c->err : g_error) = msg;
+    return st;
+}
+
+tvs_status hip_fail(tvs_ctx *c, hipError_t e, const char *what) {
+    const tvs_status st =
+        (e == hipErrorOutOfMemory) ? TVS_ERR_OUT_OF_MEMORY : TVS_ERR_HIP;
+    return fail(c, st, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define TVS_HIP(c, expr, what)                                  \
+    do {                                                        \
+        hipError_t _e = (expr);                                 \
+        if (_e != hipSuccess) return hip_fail((c), _e, (what)); \
+    } while (0)
+
+template <typename T>
+hipError_t dev_alloc(T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T));
+}
+
+template <typename T>
+void dev_free(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// Bounds on |dh| = |fl32(h - fl32(hp + obs))| over every pair of integer
+// heights h, hp in [lo, hi] (a superset of the DEM's values).  The fast
+// divide needs every non-zero |dh| in [2^-40, 2^40].
+bool fast_divide_holds(int64_t lo, int64_t hi, float obs, float *min_nz,
+                       float *max_abs) {
+    if (hi < lo || hi - lo > (int64_t(1) << 20)) return false;
+    float mn = INFINITY, mx = 0.0f;
+    for (int64_t hp = lo; hp <= hi; ++hp) {
+        const float h0 = static_cast<float>(hp) + obs;
+        if (!std::isfinite(h0)) return false;
+        mx = std::max(mx, std::fabs(static_cast<float>(hi) - h0));
+        mx = std::max(mx, std::fabs(static_cast<float>(lo) - h0));
+        const int64_t f = static_cast<int64_t>(std::floor(static_cast<double>(h0)));
+        const int64_t cand[6] = {f - 1, f, f + 1, f + 2, lo, hi};
+        for (int64_t h : cand) {
+            if (h < lo || h > hi) continue;
+            const float dh = static_cast<float>(h) - h0;
+            if (dh != 0.0f) mn = std::min(mn, std::fabs(dh));
+        }
+    }
+    *min_nz = mn;
+    *max_abs = mx;
+    return mn >= 0x1p-40f && mx <= 0x1p40f;
+}
+
+tvs_status check_band_geometry(tvs_ctx *c) {
+    const tvs_geometry &g = c->geo;
+    const uint32_t middle = g.band_samples / 2u;
+    const uint32_t pov_id = g.dem_size / 2u;
+    // band_of_sight.rs:103 (usize underflow), 113 (window past the DEM),
+    // 162 (band_id + 1 past the samples), 165 (i past band_size - 1).
+    if (middle > pov_id || pov_id - middle + g.band_samples > g.dem_size ||
+        2u * g.band_size >= g.band_samples || g.band_size < 2u ||
+        2u * g.band_size - middle > g.band_deltas_size)
+        return fail(c, TVS_ERR_BAND_TOO_SHORT,
+                    "band template does not fit (band_size " + std::to_string(g.band_size) +
+                        ", band_samples " + std::to_string(g.band_samples) +
+                        "): the reference panics here (band_of_sight.rs:103-168)");
+    if (g.band_samples > tvs::max_band_samples())
+        return fail(c, TVS_ERR_INVALID_ARGUMENT,
+                    "band_samples " + std::to_string(g.band_samples) +
+                        " exceeds the device band compile limit " +
+                        std::to_string(tvs::max_band_samples()));
+    return TVS_OK;
+}
+
+// kernel.rs:131, 157-164: every band point of every observer must index the
+// elevation array.
+tvs_status check_bounds(tvs_ctx *c, uint32_t slot) {
+    const tvs_geometry &g = c->geo;
+    const tvs::SectorMeta &m = c->meta_host[slot];
+    if (g.computable_points_count == 0) return TVS_OK;
+    const int64_t N = g.computable_points_count, T = g.tvs_width, L = g.max_los_as_points,
+                  w = g.dem_width;
+    const int64_t pov_min = L * w + L;
+    const int64_t pov_max = ((N - 1) / T + L) * w + ((N - 1) % T + L);
+    const int64_t lo = std::min<int64_t>(m.min_off, -static_cast<int64_t>(m.max_off));
+    const int64_t hi = std::max<int64_t>(m.max_off, -static_cast<int64_t>(m.min_off));
+    if (pov_min + lo < 0 || pov_max + hi > w * w - 1)
+        return fail(c, TVS_ERR_BAND_OUT_OF_DEM,
+                    "sector slot " + std::to_string(slot) +
+                        ": band leaves the elevation array (kernel.rs:164 would panic)");
+    return TVS_OK;
+}
+
+tvs_status ensure_sort_scratch(tvs_ctx *c) {
+    tvs::SortScratch &s = c->sort;
+    const size_t n = static_cast<size_t>(c->geo.dem_size);
+    if (s.n == n && s.key_a) return TVS_OK;
+    const uint32_t tiles = static_cast<uint32_t>((n + 4095) / 4096);
+    TVS_HIP(c, dev_alloc(&s.key_a, n), "sort keys");
+    TVS_HIP(c, dev_alloc(&s.key_b, n), "sort keys");
+    TVS_HIP(c, dev_alloc(&s.id_a, n), "sort ids");
+    TVS_HIP(c, dev_alloc(&s.id_b, n), "sort ids");
+    TVS_HIP(c, dev_alloc(&s.sight_map, n), "sight map");
+    TVS_HIP(c, dev_alloc(&s.dist_f32, n), "axes distances");
+    TVS_HIP(c, dev_alloc(&s.hist, static_cast<size_t>(256) * tiles), "sort histogram");
+    s.n = n;
+    s.tiles = tiles;
+    return TVS_OK;
+}
+
+void free_sort_scratch(tvs::SortScratch &s) {
+    dev_free(s.key_a);
+    dev_free(s.key_b);
+    dev_free(s.id_a);
+    dev_free(s.id_b);
+    dev_free(s.sight_map);
+    dev_free(s.dist_f32);
+    dev_free(s.hist);
+    s.n = 0;
+}
+
+tvs_status read_meta(tvs_ctx *c, uint32_t first, uint32_t count) {
+    TVS_HIP(c, hipStreamSynchronize(c->stream), "band tables");
+    TVS_HIP(c, hipMemcpy(c->meta_host.data() + first, c->tab.meta + first,
+                         count * sizeof(tvs::SectorMeta), hipMemcpyDeviceToHost),
+            "band table metadata");
+    for (uint32_t s = first; s < first + count; ++s) {
+        const tvs_status st = check_bounds(c, s);
+        if (st != TVS_OK) {
+            c->band_ready[s] = 0;
+            return st;
+        }
+        c->band_ready[s] = 1;
+    }
+    return TVS_OK;
+}
+
+tvs_status ensure_ring(tvs_ctx *c, size_t slots) {
+    if (c->ring_slots >= slots && c->ring) return TVS_OK;
+    dev_free(c->ring);
+    c->ring_slots = 0;
+    const size_t per = 2ull * c->geo.computable_points_count * c->rings_per_band;
+    TVS_HIP(c, dev_alloc(&c->ring, per * slots), "ring buffer");
+    c->ring_slots = slots;
+    return TVS_OK;
+}
+
+tvs_status upload_common(tvs_ctx *c, bool integral, int64_t lo, int64_t hi) {
+    float mn = 0.0f, mx = 0.0f;
+    c->fast_div_ok = integral &&
+                     fast_divide_holds(lo, hi, c->params.observer_height, &mn, &mx);
+    c->min_abs_dh = mn;
+    c->max_abs_dh = mx;
+    return TVS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *tvs_status_string(tvs_status st) {
+    switch (st) {
+        case TVS_OK: return "ok";
+        case TVS_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case TVS_ERR_LOS_TOO_LONG: return "max line of sight too long";
+        case TVS_ERR_BAND_TOO_SHORT: return "band template does not fit";
+        case TVS_ERR_BAND_OUT_OF_DEM: return "band leaves the DEM";
+        case TVS_ERR_SIZE: return "size mismatch";
+        case TVS_ERR_STATE: return "call out of order";
+        case TVS_ERR_NO_DEVICE: return "no HIP device";
+        case TVS_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case TVS_ERR_HIP: return "HIP runtime error";
+        case TVS_ERR_IO: return "I/O error";
+    }
+    return "unknown status";
+}
+
+const char *tvs_last_error(const tvs_ctx *c) {
+    return c ? c->err.c_str() : g_error.c_str();
+}
+
+tvs_status tvs_create(const tvs_params *p, tvs_ctx **out) {
+    if (!p || !out) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    tvs_params params = *p;
+    if (params.max_line_of_sight == 0)
+        params.max_line_of_sight =
+            tvs_default_max_line_of_sight(params.dem_width, static_cast<double>(params.scale));
+    tvs_geometry g;
+    tvs_status st = tvs::dem_geometry(params.dem_width, params.scale,
+                                      params.max_line_of_sight, &g);
+    if (st == TVS_ERR_LOS_TOO_LONG)
+        return fail(nullptr, st,
+                    "The maximum line of sight (" + std::to_string(params.max_line_of_sight) +
+                        "m) is longer than the maximum distance that any point can "
+                        "completely see (" +
+                        std::to_string(static_cast<double>(params.dem_width / 2u) *
+                                       static_cast<double>(params.scale)) +
+                        "m).");
+    if (st != TVS_OK) return fail(nullptr, st, "invalid DEM width or scale");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(nullptr, TVS_ERR_NO_DEVICE, "no HIP device visible");
+    tvs_ctx *c = new (std::nothrow) tvs_ctx();
+    if (!c) return fail(nullptr, TVS_ERR_OUT_OF_MEMORY, "host allocation");
+    c->params = params;
+    c->geo = g;
+    c->rings_per_band = tvs_ring_count_per_band(params.rings_per_km, params.max_line_of_sight);
+    hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    for (int i = 0; i < 6 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
+    const uint32_t N = g.computable_points_count;
+    if (e == hipSuccess) e = dev_alloc(&c->cum, N);
+    if (e == hipSuccess) e = dev_alloc(&c->cum64, N);
+    if (e == hipSuccess) e = dev_alloc(&c->overflow, 1);
+    c->tab.stride = std::max<uint32_t>(64u, (g.band_deltas_size + 63u) / 64u * 64u);
+    const size_t tab_n = static_cast<size_t>(tvs::kBandSlots) * c->tab.stride;
+    if (e == hipSuccess) e = dev_alloc(&c->tab.delta, tab_n);
+    if (e == hipSuccess) e = dev_alloc(&c->tab.dist, tab_n);
+    if (e == hipSuccess) e = dev_alloc(&c->tab.off, tab_n);
+    if (e == hipSuccess) e = dev_alloc(&c->tab.rcp, tab_n);
+    if (e == hipSuccess) e = dev_alloc(&c->tab.curv, tab_n);
+    if (e == hipSuccess) e = dev_alloc(&c->tab.wt, tab_n);
+    if (e == hipSuccess) e = dev_alloc(&c->tab.meta, tvs::kBandSlots);
+    if (e == hipSuccess) e = hipMemsetAsync(c->cum, 0, sizeof(float) * std::max(N, 1u), c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->cum64, 0, sizeof(double) * std::max(N, 1u), c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->overflow, 0, sizeof(unsigned long long), c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        const tvs_status s2 = hip_fail(nullptr, e, "tvs_create");
+        tvs_destroy(c);
+        return s2;
+    }
+    c->band_ready.assign(tvs::kBandSlots, 0);
+    c->meta_host.assign(tvs::kBandSlots, tvs::SectorMeta{});
+    *out = c;
+    return TVS_OK;
+}
+
+tvs_status tvs_destroy(tvs_ctx *c) {
+    if (!c) return TVS_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    dev_free(c->elev_f32);
+    dev_free(c->elev_i16);
+    dev_free(c->tab.delta);
+    dev_free(c->tab.dist);
+    dev_free(c->tab.off);
+    dev_free(c->tab.rcp);
+    dev_free(c->tab.curv);
+    dev_free(c->tab.wt);
+    dev_free(c->tab.meta);
+    dev_free(c->cum);
+    dev_free(c->cum64);
+    dev_free(c->partial);
+    dev_free(c->ring);
+    dev_free(c->overflow);
+    free_sort_scratch(c->sort);
+    for (auto &ev : c->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return TVS_OK;
+}
+
+tvs_status tvs_get_geometry(const tvs_ctx *c, tvs_geometry *out) {
+    if (!c || !out) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null argument");
+    *out = c->geo;
+    return TVS_OK;
+}
+
+tvs_status tvs_upload_dem_f32(tvs_ctx *c, const float *elev, size_t count) {
+    if (!c || !elev) return fail(c, TVS_ERR_INVALID_ARGUMENT, "null argument");
+    if (count != c->geo.dem_size)
+        return fail(c, TVS_ERR_SIZE, "elevation count " + std::to_string(count) +
+                                         " != width^2 " + std::to_string(c->geo.dem_size));
+    if (!c->elev_f32) TVS_HIP(c, dev_alloc(&c->elev_f32, count), "elevations");
+    TVS_HIP(c, hipMemcpyAsync(c->elev_f32, elev, count * sizeof(float),
+                              hipMemcpyHostToDevice, c->stream),
+            "elevation upload");
+    TVS_HIP(c, hipStreamSynchronize(c->stream), "elevation upload");
+    dev_free(c->elev_i16);
+    c->has_i16 = false;
+    c->has_f32 = true;
+    bool integral = true;
+    double lo = INFINITY, hi = -INFINITY;
+    for (size_t i = 0; i < count && integral; ++i) {
+        const float v = elev[i];
+        if (!(std::fabs(v) < 16777216.0f) || v != std::floor(v)) integral = false;
+        lo = std::min(lo, static_cast<double>(v));
+        hi = std::max(hi, static_cast<double>(v));
+    }
+    return upload_common(c, integral, integral ? static_cast<int64_t>(lo) : 0,
+                         integral ? static_cast<int64_t>(hi) : -1);
+}
+
+tvs_status tvs_upload_dem_i16(tvs_ctx *c, const int16_t *elev, size_t count) {
+    if (!c || !elev) return fail(c, TVS_ERR_INVALID_ARGUMENT, "null argument");
+    if (count != c->geo.dem_size)
+        return fail(c, TVS_ERR_SIZE, "elevation count " + std::to_string(count) +
+                                         " != width^2 " + std::to_string(c->geo.dem_size));
+    if (!c->elev_i16) TVS_HIP(c, dev_alloc(&c->elev_i16, count), "elevations");
+    if (!c->elev_f32) TVS_HIP(c, dev_alloc(&c->elev_f32, count), "elevations");
+    TVS_HIP(c, hipMemcpyAsync(c->elev_i16, elev, count * sizeof(int16_t),
+                              hipMemcpyHostToDevice, c->stream),
+            "elevation upload");
+    TVS_HIP(c, tvs::launch_widen_i16(c->stream, c->elev_i16, c->elev_f32, count),
+            "elevation widen");
+    TVS_HIP(c, hipStreamSynchronize(c->stream), "elevation upload");
+    c->has_i16 = true;
+    c->has_f32 = true;
+    int64_t lo = 32767, hi = -32768;
+    for (size_t i = 0; i < count; ++i) {
+        lo = std::min<int64_t>(lo, elev[i]);
+        hi = std::max<int64_t>(hi, elev[i]);
+    }
+    return upload_common(c, true, lo, hi);
+}
+
+tvs_status tvs_compute_bands(tvs_ctx *c, uint32_t first, uint32_t count) {
+    if (!c) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null context");
+    if (first >= tvs::kSectorSteps || count > tvs::kSectorSteps - first)
+        return fail(c, TVS_ERR_INVALID_ARGUMENT, "sector range outside 0..180");
+    if (count == 0) return TVS_OK;
+    tvs_status st = check_band_geometry(c);
+    if (st != TVS_OK) return st;
+    if ((st = ensure_sort_scratch(c)) != TVS_OK) return st;
+    TVS_HIP(c, hipEventRecord(c->ev[4], c->stream), "event");
+    for (uint32_t s = first; s < first + count; ++s) {
+        TVS_HIP(c, tvs::launch_axes(c, s, false), "axes sort");
+        TVS_HIP(c, tvs::launch_band_compile(c, s), "band compile");
+    }
+    TVS_HIP(c, tvs::launch_step_tables(c, first, count), "step tables");
+    TVS_HIP(c, hipEventRecord(c->ev[5], c->stream), "event");
+    TVS_HIP(c, hipEventSynchronize(c->ev[5]), "band computation");
+    float ms = 0.0f;
+    TVS_HIP(c, hipEventElapsedTime(&ms, c->ev[4], c->ev[5]), "event time");
+    c->timing.bands_ms = ms;
+    return read_meta(c, first, count);
+}
+
+tvs_status tvs_compute_axes(tvs_ctx *c, uint32_t sector, float *distances,
+                            uint32_t *sight_ordered_map, uint32_t *sector_ordered) {
+    if (!c) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null context");
+    if (sector >= tvs::kSectorSteps)
+        return fail(c, TVS_ERR_INVALID_ARGUMENT, "sector outside 0..180");
+    tvs_status st = ensure_sort_scratch(c);
+    if (st != TVS_OK) return st;
+    TVS_HIP(c, tvs::launch_axes(c, sector, true), "axes sort");
+    TVS_HIP(c, hipStreamSynchronize(c->stream), "axes sort");
+    const size_t n = c->geo.dem_size;
+    if (distances)
+        TVS_HIP(c, hipMemcpy(distances, c->sort.dist_f32, n * sizeof(float), hipMemcpyDeviceToHost),
+                "axes readback");
+    if (sight_ordered_map)
+        TVS_HIP(c, hipMemcpy(sight_ordered_map, c->sort.sight_map, n * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost),
+                "axes readback");
+    if (sector_ordered)
+        TVS_HIP(c, hipMemcpy(sector_ordered, c->sort.id_a, n * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost),
+                "axes readback");
+    return TVS_OK;
+}
+
+static tvs_status set_band_slot(tvs_ctx *c, uint32_t slot, const int32_t *deltas,
+                                const float *dists, size_t n) {
+    if (!deltas || !dists) return fail(c, TVS_ERR_INVALID_ARGUMENT, "null band data");
+    if (n != c->geo.band_deltas_size)
+        return fail(c, TVS_ERR_SIZE, "band data length " + std::to_string(n) +
+                                         " != band_size - 1 = " +
+                                         std::to_string(c->geo.band_deltas_size));
+    const size_t tb = static_cast<size_t>(slot) * c->tab.stride;
+    if (n) {
+        TVS_HIP(c, hipMemcpyAsync(c->tab.delta + tb, deltas, n * sizeof(int32_t),
+                                  hipMemcpyHostToDevice, c->stream),
+                "band upload");
+        TVS_HIP(c, hipMemcpyAsync(c->tab.dist + tb, dists, n * sizeof(float),
+                                  hipMemcpyHostToDevice, c->stream),
+                "band upload");
+    }
+    TVS_HIP(c, tvs::launch_step_tables(c, slot, 1), "step tables");
+    return read_meta(c, slot, 1);
+}
+
+tvs_status tvs_set_band(tvs_ctx *c, uint32_t sector, const int32_t *deltas,
+                        const float *dists, size_t n) {
+    if (!c) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null context");
+    if (sector >= tvs::kSectorSteps)
+        return fail(c, TVS_ERR_INVALID_ARGUMENT, "sector outside 0..180");
+    return set_band_slot(c, sector, deltas, dists, n);
+}
+
+tvs_status tvs_get_band(tvs_ctx *c, uint32_t sector, int32_t *deltas, float *dists,
+                        size_t n) {
+    if (!c) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null context");
+    if (sector >= tvs::kSectorSteps)
+        return fail(c, TVS_ERR_INVALID_ARGUMENT, "sector outside 0..180");
+    if (!c->band_ready[sector]) return fail(c, TVS_ERR_STATE, "sector has no band data");
+    if (n != c->geo.band_deltas_size) return fail(c, TVS_ERR_SIZE, "band data length");
+    TVS_HIP(c, hipStreamSynchronize(c->stream), "sync");
+    const size_t tb = static_cast<size_t>(sector) * c->tab.stride;
+    if (n && deltas)
+        TVS_HIP(c, hipMemcpy(deltas, c->tab.delta + tb, n * sizeof(int32_t), hipMemcpyDeviceToHost),
+                "band readback");
+    if (n && dists)
+        TVS_HIP(c, hipMemcpy(dists, c->tab.dist + tb, n * sizeof(float), hipMemcpyDeviceToHost),
+                "band readback");
+    return TVS_OK;
+}
+
+static tvs_status begin_capture(tvs_ctx *c, uint32_t capture) {
+    TVS_HIP(c, hipMemsetAsync(c->overflow, 0, sizeof(unsigned long long), c->stream),
+            "overflow reset");
+    c->ring_valid = 0;
+    if (capture == 0) return TVS_OK;
+    tvs_status st = ensure_ring(c, capture);
+    if (st != TVS_OK) return st;
+    const size_t per = 2ull * c->geo.computable_points_count * c->rings_per_band;
+    TVS_HIP(c, tvs::launch_fill_u32(c->stream, c->ring, per * capture, 0u), "ring clear");
+    return TVS_OK;
+}
+
+tvs_status tvs_run_sectors(tvs_ctx *c, uint32_t first, uint32_t count, tvs_method method) {
+    if (!c) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null context");
+    if (first >= tvs::kSectorSteps || count > tvs::kSectorSteps - first)
+        return fail(c, TVS_ERR_INVALID_ARGUMENT, "sector range outside 0..180");
+    if (method != TVS_METHOD_PLAIN && method != TVS_METHOD_BATCHED)
+        return fail(c, TVS_ERR_INVALID_ARGUMENT, "unknown method");
+    if (!c->has_f32) return fail(c, TVS_ERR_STATE, "no elevations uploaded");
+    for (uint32_t s = first; s < first + count; ++s)
+        if (!c->band_ready[s])
+            return fail(c, TVS_ERR_STATE, "sector " + std::to_string(s) + " has no band data");
+    const uint32_t N = c->geo.computable_points_count;
+    const uint32_t capture = std::min(c->params.ring_capture_sectors, count);
+    tvs_status st = begin_capture(c, capture);
+    if (st != TVS_OK) return st;
+    c->timing = tvs_timing{0.0f, 0.0f, c->timing.bands_ms, 0u,
+                           static_cast<uint64_t>(count) * 2ull * N * c->geo.band_deltas_size};
+    if (count == 0 || N == 0) return TVS_OK;
+    const size_t per = 2ull * N * c->rings_per_band;
+    if (method == TVS_METHOD_PLAIN) {
+        TVS_HIP(c, hipEventRecord(c->ev[0], c->stream), "event");
+        for (uint32_t r = 0; r < count; ++r) {
+            uint32_t *ring = r < capture ? c->ring + per * r : nullptr;
+            TVS_HIP(c, tvs::launch_walk_plain(c, first + r, ring), "plain walk");
+        }
+        TVS_HIP(c, hipEventRecord(c->ev[1], c->stream), "event");
+        TVS_HIP(c, hipEventRecord(c->ev[2], c->stream), "event");
+        c->timing.walk_launches = count;
+    } else {
+        const size_t need = static_cast<size_t>(count);
+        if (c->partial_cap < need) {
+            dev_free(c->partial);
+            c->partial_cap = 0;
+            TVS_HIP(c, dev_alloc(&c->partial, need * 2ull * N), "band surfaces");
+            c->partial_cap = need;
+        }
+        TVS_HIP(c, hipEventRecord(c->ev[0], c->stream), "event");
+        TVS_HIP(c, tvs::launch_walk_batched(c, first, count, capture), "batched walk");
+        TVS_HIP(c, hipEventRecord(c->ev[1], c->stream), "event");
+        TVS_HIP(c, tvs::launch_reduce(c, count), "sector reduction");
+        TVS_HIP(c, hipEventRecord(c->ev[2], c->stream), "event");
+        c->timing.walk_launches = 1;
+    }
+    c->ring_valid = capture;
+    return TVS_OK;
+}
+
+tvs_status tvs_run_sector_with_band(tvs_ctx *c, const int32_t *deltas, const float *dists,
+                                    size_t n, float *cum_out, size_t n_out) {
+    if (!c) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null context");
+    if (!c->has_f32) return fail(c, TVS_ERR_STATE, "no elevations uploaded");
+    const uint32_t N = c->geo.computable_points_count;
+    if (cum_out && n_out != N) return fail(c, TVS_ERR_SIZE, "surface buffer length");
+    tvs_status st = set_band_slot(c, tvs::kScratchSector, deltas, dists, n);
+    if (st != TVS_OK) return st;
+    const uint32_t capture = std::min(c->params.ring_capture_sectors, 1u);
+    if ((st = begin_capture(c, capture)) != TVS_OK) return st;
+    if (N) {
+        TVS_HIP(c, hipEventRecord(c->ev[0], c->stream), "event");
+        TVS_HIP(c, tvs::launch_walk_plain(c, tvs::kScratchSector, capture ? c->ring : nullptr),
+                "plain walk");
+        TVS_HIP(c, hipEventRecord(c->ev[1], c->stream), "event");
+        TVS_HIP(c, hipEventRecord(c->ev[2], c->stream), "event");
+    }
+    c->ring_valid = capture;
+    c->timing = tvs_timing{0.0f, 0.0f, c->timing.bands_ms, N ? 1u : 0u,
+                           2ull * N * c->geo.band_deltas_size};
+    if (cum_out) return tvs_read_surfaces(c, cum_out, n_out);
+    return TVS_OK;
+}
+
+tvs_status tvs_reset_surfaces(tvs_ctx *c) {
+    if (!c) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null context");
+    const size_t N = std::max(c->geo.computable_points_count, 1u);
+    TVS_HIP(c, hipMemsetAsync(c->cum, 0, N * sizeof(float), c->stream), "reset");
+    TVS_HIP(c, hipMemsetAsync(c->cum64, 0, N * sizeof(double), c->stream), "reset");
+    return TVS_OK;
+}
+
+tvs_status tvs_read_surfaces(tvs_ctx *c, float *out, size_t n) {
+    if (!c || !out) return fail(c, TVS_ERR_INVALID_ARGUMENT, "null argument");
+    if (n != c->geo.computable_points_count) return fail(c, TVS_ERR_SIZE, "surface length");
+    TVS_HIP(c, hipStreamSynchronize(c->stream), "sync");
+    if (n) TVS_HIP(c, hipMemcpy(out, c->cum, n * sizeof(float), hipMemcpyDeviceToHost), "readback");
+    return TVS_OK;
+}
+
+tvs_status tvs_read_surfaces_f64(tvs_ctx *c, double *out, size_t n) {
+    if (!c || !out) return fail(c, TVS_ERR_INVALID_ARGUMENT, "null argument");
+    if (n != c->geo.computable_points_count) return fail(c, TVS_ERR_SIZE, "surface length");
+    TVS_HIP(c, hipStreamSynchronize(c->stream), "sync");
+    if (n) TVS_HIP(c, hipMemcpy(out, c->cum64, n * sizeof(double), hipMemcpyDeviceToHost), "readback");
+    return TVS_OK;
+}
+
+tvs_status tvs_read_rings(tvs_ctx *c, uint32_t slot, uint32_t *out, size_t n) {
+    if (!c || !out) return fail(c, TVS_ERR_INVALID_ARGUMENT, "null argument");
+    if (slot >= c->ring_valid) return fail(c, TVS_ERR_STATE, "ring slot not captured by the last run");
+    const size_t per = 2ull * c->geo.computable_points_count * c->rings_per_band;
+    if (n != per) return fail(c, TVS_ERR_SIZE, "ring buffer length != 2N*R");
+    TVS_HIP(c, hipStreamSynchronize(c->stream), "sync");
+    if (n) TVS_HIP(c, hipMemcpy(out, c->ring + per * slot, n * sizeof(uint32_t), hipMemcpyDeviceToHost),
+                   "ring readback");
+    return TVS_OK;
+}
+
+tvs_status tvs_ring_overflow_count(tvs_ctx *c, uint64_t *out) {
+    if (!c || !out) return fail(c, TVS_ERR_INVALID_ARGUMENT, "null argument");
+    TVS_HIP(c, hipStreamSynchronize(c->stream), "sync");
+    unsigned long long v = 0;
+    TVS_HIP(c, hipMemcpy(&v, c->overflow, sizeof v, hipMemcpyDeviceToHost), "readback");
+    *out = v;
+    return TVS_OK;
+}
+
+tvs_status tvs_sync(tvs_ctx *c) {
+    if (!c) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null context");
+    TVS_HIP(c, hipStreamSynchronize(c->stream), "sync");
+    return TVS_OK;
+}
+
+tvs_status tvs_get_timing(tvs_ctx *c, tvs_timing *out) {
+    if (!c || !out) return fail(c, TVS_ERR_INVALID_ARGUMENT, "null argument");
+    if (c->timing.walk_launches) {
+        TVS_HIP(c, hipEventSynchronize(c->ev[2]), "event");
+        float ms = 0.0f;
+        TVS_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]), "event time");
+        c->timing.walk_ms = ms;
+        TVS_HIP(c, hipEventElapsedTime(&ms, c->ev[1], c->ev[2]), "event time");
+        c->timing.reduce_ms = ms;
+    }
+    *out = c->timing;
+    return TVS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,126 @@
+// engine.hpp -- internal declarations shared by the engine's translation
+// units (device buffers, per-sector step tables, kernel launchers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/tvs.h"
+
+namespace tvs {
+
+// kernel.rs:13, 17, 21
+constexpr float kMaxAngle = -2000.0f;
+constexpr float kTanOneRad = 0.0174533f;
+constexpr float kEarthRadiusSquared = 12742000.0f;
+// axes.rs:35, 47
+constexpr uint32_t kSectorSteps = 180;
+constexpr float kSectorShift = 0.001f;
+// One extra band-table slot holds band data handed in by
+// tvs_run_sector_with_band.
+constexpr uint32_t kScratchSector = kSectorSteps;
+constexpr uint32_t kBandSlots = kSectorSteps + 1;
+
+// Steps of one sector's walk are split in three ranges, each a uniform
+// (whole-grid) property of the sector:
+//   [0, n_fast)            regular steps: d normal, the fast divide holds
+//   [n_fast, tail_start)   anything else: per-step choice of divide
+//   [tail_start, n_steps)  the zero tail of band_of_sight.rs:67-68 (delta 0,
+//                          distance +0): the point and the angle repeat.
+struct SectorMeta {
+    int32_t min_off;     // minimum cumulative offset over the steps
+    int32_t max_off;     // maximum cumulative offset over the steps
+    uint32_t n_fast;
+    uint32_t tail_start;
+    uint32_t status;     // non-zero: band compile hit a reference panic site
+    uint32_t pad[3];
+};
+
+// Device-resident per-sector step tables (structure of arrays, `stride`
+// entries per sector slot).  Derived from the reference's band data
+// (deltas, distances) only; the per-step constants are shared by all 2N
+// bands of the sector.
+struct StepTables {
+    int32_t *delta = nullptr;  // band_deltas            (band_of_sight.rs:165)
+    float *dist = nullptr;     // band_distances         (band_of_sight.rs:168)
+    int32_t *off = nullptr;    // inclusive prefix sum of delta
+    float *rcp = nullptr;      // RN(1 / dist)
+    float *curv = nullptr;     // RN(dist / 12742000)    (kernel.rs:175)
+    float *wt = nullptr;       // RN(dist * 0.0174533)   (kernel.rs:216)
+    SectorMeta *meta = nullptr;
+    uint32_t stride = 0;
+};
+
+// Scratch for the device Axes::compute (keys, ids, radix-sort ping-pong,
+// histograms).
+struct SortScratch {
+    uint32_t *key_a = nullptr, *key_b = nullptr;
+    uint32_t *id_a = nullptr, *id_b = nullptr;
+    uint32_t *sight_map = nullptr;   // axes.rs:121-124
+    uint32_t *sector_ids = nullptr;  // axes.rs:127-129
+    uint32_t *hist = nullptr;
+    float *dist_f32 = nullptr;       // axes.distances (axes.rs:187-189)
+    size_t n = 0;
+    uint32_t tiles = 0;
+};
+
+}  // namespace tvs
+
+struct tvs_ctx {
+    tvs_geometry geo{};
+    tvs_params params{};
+    uint32_t rings_per_band = 0;   // R
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[6] = {};
+
+    float *elev_f32 = nullptr;
+    int16_t *elev_i16 = nullptr;
+    bool has_f32 = false, has_i16 = false;
+    // Height-difference bounds proven on the host for the fast divide.
+    bool fast_div_ok = false;
+    float min_abs_dh = 0.0f, max_abs_dh = 0.0f;
+
+    tvs::StepTables tab;
+    std::vector<uint8_t> band_ready;   // per slot: band data resident
+    std::vector<tvs::SectorMeta> meta_host;
+
+    float *cum = nullptr;      // f32[N]  cumulative surfaces
+    double *cum64 = nullptr;   // f64[N]
+    float *partial = nullptr;  // f32[sectors][2][N] (batched method)
+    size_t partial_cap = 0;    // sectors the partial buffer holds
+    uint32_t *ring = nullptr;  // u32[capture][2N][R]
+    size_t ring_slots = 0;
+    uint32_t ring_valid = 0;   // slots filled by the last run
+    unsigned long long *overflow = nullptr;
+
+    tvs::SortScratch sort;
+    tvs_timing timing{};
+    std::string err;
+};
+
+namespace tvs {
+
+// ---- launchers (bands.hip) ----
+hipError_t launch_axes(tvs_ctx *c, uint32_t sector, bool want_distances);
+hipError_t launch_band_compile(tvs_ctx *c, uint32_t sector);
+hipError_t launch_step_tables(tvs_ctx *c, uint32_t first_slot, uint32_t count);
+
+// ---- launchers (walk.hip) ----
+hipError_t launch_walk_plain(tvs_ctx *c, uint32_t slot, uint32_t *ring_out);
+hipError_t launch_walk_batched(tvs_ctx *c, uint32_t first_slot,
+                               uint32_t count, uint32_t capture_slots);
+hipError_t launch_reduce(tvs_ctx *c, uint32_t count);
+hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, size_t n, uint32_t v);
+hipError_t launch_widen_i16(hipStream_t s, const int16_t *in, float *out,
+                            size_t n);
+
+// ---- host geometry (geometry.cpp) ----
+tvs_status dem_geometry(uint32_t width, float scale, uint32_t los,
+                        tvs_geometry *out);
+void sector_sin_cos(uint32_t sector, bool sector_axis, double *s, double *c);
+
+}  // namespace tvs

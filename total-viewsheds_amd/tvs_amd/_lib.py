@@ -1,0 +1,289 @@
+"""ctypes binding of libtvs_hip.so (the C ABI declared in include/tvs.h).
+
+The library must have been built (``make -C total-viewsheds_amd`` or
+``__graft_entry__.build()``); loading fails loudly otherwise -- there is no
+CPU fallback in the engine.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libtvs_hip.so")
+HEADER_PATH = os.path.join(os.path.dirname(PKG_DIR), "include", "tvs.h")
+
+# tvs_status
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_LOS_TOO_LONG = 2
+ERR_BAND_TOO_SHORT = 3
+ERR_BAND_OUT_OF_DEM = 4
+ERR_SIZE = 5
+ERR_STATE = 6
+ERR_NO_DEVICE = 7
+ERR_OUT_OF_MEMORY = 8
+ERR_HIP = 9
+ERR_IO = 10
+
+# tvs_method
+METHOD_PLAIN = 0
+METHOD_BATCHED = 1
+METHODS = {"plain": METHOD_PLAIN, "batched": METHOD_BATCHED}
+
+
+class Geometry(ctypes.Structure):
+    _fields_ = [
+        ("dem_width", ctypes.c_uint32),
+        ("dem_size", ctypes.c_uint32),
+        ("scale", ctypes.c_float),
+        ("max_line_of_sight", ctypes.c_uint32),
+        ("max_los_as_points", ctypes.c_uint32),
+        ("computable_points_count", ctypes.c_uint32),
+        ("tvs_width", ctypes.c_uint32),
+        ("band_size", ctypes.c_uint32),
+        ("band_deltas_size", ctypes.c_uint32),
+        ("band_samples", ctypes.c_uint32),
+        ("real_steps", ctypes.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("dem_width", ctypes.c_uint32),
+        ("scale", ctypes.c_float),
+        ("max_line_of_sight", ctypes.c_uint32),
+        ("rings_per_km", ctypes.c_float),
+        ("observer_height", ctypes.c_float),
+        ("ring_capture_sectors", ctypes.c_uint32),
+    ]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [
+        ("walk_ms", ctypes.c_float),
+        ("reduce_ms", ctypes.c_float),
+        ("bands_ms", ctypes.c_float),
+        ("walk_launches", ctypes.c_uint32),
+        ("point_steps", ctypes.c_uint64),
+    ]
+
+
+class TvsError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{message} (status {status})")
+        self.status = status
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is not built: run `make -C {PKG_DIR}` "
+            "(or __graft_entry__.build()); the engine has no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    st = ctypes.c_int
+    P = ctypes.POINTER
+    sig = {
+        "tvs_dem_geometry": (st, [ctypes.c_uint32, ctypes.c_float, ctypes.c_uint32, P(Geometry)]),
+        "tvs_default_max_line_of_sight": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_double]),
+        "tvs_ring_count_per_band": (ctypes.c_uint32, [ctypes.c_float, ctypes.c_uint32]),
+        "tvs_create": (st, [P(Params), P(vp)]),
+        "tvs_destroy": (st, [vp]),
+        "tvs_get_geometry": (st, [vp, P(Geometry)]),
+        "tvs_upload_dem_f32": (st, [vp, vp, ctypes.c_size_t]),
+        "tvs_upload_dem_i16": (st, [vp, vp, ctypes.c_size_t]),
+        "tvs_compute_bands": (st, [vp, ctypes.c_uint32, ctypes.c_uint32]),
+        "tvs_set_band": (st, [vp, ctypes.c_uint32, vp, vp, ctypes.c_size_t]),
+        "tvs_get_band": (st, [vp, ctypes.c_uint32, vp, vp, ctypes.c_size_t]),
+        "tvs_compute_axes": (st, [vp, ctypes.c_uint32, vp, vp, vp]),
+        "tvs_run_sectors": (st, [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]),
+        "tvs_run_sector_with_band": (st, [vp, vp, vp, ctypes.c_size_t, vp, ctypes.c_size_t]),
+        "tvs_reset_surfaces": (st, [vp]),
+        "tvs_read_surfaces": (st, [vp, vp, ctypes.c_size_t]),
+        "tvs_read_surfaces_f64": (st, [vp, vp, ctypes.c_size_t]),
+        "tvs_read_rings": (st, [vp, ctypes.c_uint32, vp, ctypes.c_size_t]),
+        "tvs_ring_overflow_count": (st, [vp, P(ctypes.c_uint64)]),
+        "tvs_sync": (st, [vp]),
+        "tvs_get_timing": (st, [vp, P(Timing)]),
+        "tvs_last_error": (ctypes.c_char_p, [vp]),
+        "tvs_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "arrays passed to the engine must be contiguous"
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(status: int, ctx=None) -> None:
+    if status != OK:
+        msg = load().tvs_last_error(ctx)
+        raise TvsError(status, (msg or b"").decode() or
+                       load().tvs_status_string(status).decode())
+
+
+def dem_geometry(width: int, scale: float, max_line_of_sight: int) -> Geometry:
+    g = Geometry()
+    _check(load().tvs_dem_geometry(width, scale, max_line_of_sight, ctypes.byref(g)))
+    return g
+
+
+def default_max_line_of_sight(width: int, scale: float) -> int:
+    return int(load().tvs_default_max_line_of_sight(width, scale))
+
+
+def ring_count_per_band(rings_per_km: float, max_line_of_sight: int) -> int:
+    return int(load().tvs_ring_count_per_band(rings_per_km, max_line_of_sight))
+
+
+@dataclass
+class RunTiming:
+    walk_ms: float
+    reduce_ms: float
+    bands_ms: float
+    walk_launches: int
+    point_steps: int
+
+
+class Engine:
+    """One device context (tvs_ctx): a DEM, its resident band data, its
+    cumulative surfaces and optional ring records."""
+
+    def __init__(self, dem_width: int, scale: float, max_line_of_sight: int = 0,
+                 rings_per_km: float = 5.0, observer_height: float = 1.65,
+                 ring_capture_sectors: int = 0):
+        L = load()
+        p = Params(dem_width, scale, max_line_of_sight, rings_per_km, observer_height,
+                   ring_capture_sectors)
+        h = ctypes.c_void_p()
+        _check(L.tvs_create(ctypes.byref(p), ctypes.byref(h)))
+        self._h = h
+        self.geometry = Geometry()
+        _check(L.tvs_get_geometry(self._h, ctypes.byref(self.geometry)), self._h)
+        self.rings_per_band = ring_count_per_band(rings_per_km,
+                                                  self.geometry.max_line_of_sight)
+        self.observer_height = observer_height
+
+    # -- lifetime --
+    def close(self) -> None:
+        if self._h:
+            load().tvs_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, status: int) -> None:
+        _check(status, self._h)
+
+    # -- inputs --
+    @property
+    def n_points(self) -> int:
+        return self.geometry.computable_points_count
+
+    def upload(self, elevations: np.ndarray) -> None:
+        """int16 arrays go up as int16 (the batched method reads them at 2 B
+        per step); anything else as float32."""
+        if elevations.dtype == np.int16:
+            a = np.ascontiguousarray(elevations.reshape(-1))
+            self._c(load().tvs_upload_dem_i16(self._h, _ptr(a), a.size))
+        else:
+            a = np.ascontiguousarray(elevations.reshape(-1), np.float32)
+            self._c(load().tvs_upload_dem_f32(self._h, _ptr(a), a.size))
+
+    def compute_bands(self, first: int = 0, count: int = 180) -> None:
+        self._c(load().tvs_compute_bands(self._h, first, count))
+
+    def set_band(self, sector: int, deltas: np.ndarray, dists: np.ndarray) -> None:
+        d = np.ascontiguousarray(deltas, np.int32)
+        s = np.ascontiguousarray(dists, np.float32)
+        self._c(load().tvs_set_band(self._h, sector, _ptr(d), _ptr(s), d.size))
+
+    def get_band(self, sector: int):
+        n = self.geometry.band_deltas_size
+        d = np.empty(max(n, 1), np.int32)
+        s = np.empty(max(n, 1), np.float32)
+        self._c(load().tvs_get_band(self._h, sector, _ptr(d), _ptr(s), n))
+        return d[:n], s[:n]
+
+    def compute_axes(self, sector: int):
+        n = self.geometry.dem_size
+        dist = np.empty(n, np.float32)
+        sight = np.empty(n, np.uint32)
+        order = np.empty(n, np.uint32)
+        self._c(load().tvs_compute_axes(self._h, sector, _ptr(dist), _ptr(sight), _ptr(order)))
+        return dist, sight, order
+
+    # -- walk --
+    def run(self, first: int = 0, count: int = 180, method: str = "batched") -> None:
+        self._c(load().tvs_run_sectors(self._h, first, count, METHODS[method]))
+
+    def run_sector_with_band(self, deltas: np.ndarray, dists: np.ndarray) -> np.ndarray:
+        d = np.ascontiguousarray(deltas, np.int32)
+        s = np.ascontiguousarray(dists, np.float32)
+        out = np.empty(max(self.n_points, 1), np.float32)
+        self._c(load().tvs_run_sector_with_band(self._h, _ptr(d), _ptr(s), d.size,
+                                                _ptr(out), self.n_points))
+        return out[:self.n_points]
+
+    def reset_surfaces(self) -> None:
+        self._c(load().tvs_reset_surfaces(self._h))
+
+    def surfaces(self) -> np.ndarray:
+        out = np.empty(max(self.n_points, 1), np.float32)
+        self._c(load().tvs_read_surfaces(self._h, _ptr(out), self.n_points))
+        return out[:self.n_points]
+
+    def surfaces_f64(self) -> np.ndarray:
+        out = np.empty(max(self.n_points, 1), np.float64)
+        self._c(load().tvs_read_surfaces_f64(self._h, _ptr(out), self.n_points))
+        return out[:self.n_points]
+
+    def rings(self, slot: int) -> np.ndarray:
+        n = 2 * self.n_points * self.rings_per_band
+        out = np.empty(max(n, 1), np.uint32)
+        self._c(load().tvs_read_rings(self._h, slot, _ptr(out), n))
+        return out[:n]
+
+    def ring_overflow(self) -> int:
+        v = ctypes.c_uint64(0)
+        self._c(load().tvs_ring_overflow_count(self._h, ctypes.byref(v)))
+        return int(v.value)
+
+    def sync(self) -> None:
+        self._c(load().tvs_sync(self._h))
+
+    def timing(self) -> RunTiming:
+        t = Timing()
+        self._c(load().tvs_get_timing(self._h, ctypes.byref(t)))
+        return RunTiming(t.walk_ms, t.reduce_ms, t.bands_ms, t.walk_launches, t.point_steps)
