@@ -213,6 +213,59 @@ const char *tvs_last_error(const tvs_ctx *ctx);
 /* Static name of a status code. */
 const char *tvs_status_string(tvs_status status);
 
+/* ---- host I/O around the walk ------------------------------------------ */
+
+/* `.bt` header fields (input.rs:27-70, read at input.rs:90-113). */
+typedef struct tvs_bt_header {
+    uint32_t width;
+    uint32_t height;
+    uint16_t data_size;
+    uint16_t is_float;
+    uint16_t horizontal_units;
+    uint16_t utm_zone;
+    uint16_t datum;
+    uint16_t projection_source;
+    double left, right, bottom, top;
+    float vertical_scale;
+} tvs_bt_header;
+
+/* `BinaryTerrain::read` header part (input.rs:86-113, 158-160): magic
+ * "binterr1.3", fields, data-size check. */
+tvs_status tvs_bt_read_header(const char *path, tvs_bt_header *out);
+
+/* The samples after the 256-byte header (input.rs:133-171) as f32, widening
+ * int16 exactly (main.rs:51-56).  `count` = width * height; the reference
+ * then uses them as a width x width row-major grid (main.rs:49). */
+tvs_status tvs_bt_read_f32(const char *path, float *out, size_t count);
+
+/* The int16 samples without widening (int16 files only). */
+tvs_status tvs_bt_read_i16(const char *path, int16_t *out, size_t count);
+
+/* Write a `.bt` file (header + raw little-endian samples: int16, or f32 when
+ * is_float), e.g. to hand a synthetic DEM to the CLI. */
+tvs_status tvs_bt_write(const char *path, const tvs_bt_header *header,
+                        const void *data);
+
+/* `BinaryTerrain::scale` (input.rs:178-185): metres per point. */
+double tvs_bt_scale(const tvs_bt_header *header);
+
+/* `output::png::save` (output/png.rs:5-37): 8-bit greyscale PNG, min/max
+ * normalised. */
+tvs_status tvs_write_heatmap_png(const char *path, const float *data,
+                                 uint32_t width, uint32_t height);
+
+/* Band-data cache files `<state_dir>/total-viewsheds/<w>/{band_deltas,
+ * distances}/<sector>.bin` (cache.rs:23-76): save a resident sector / load
+ * one into the context (a file of another band length is rejected with
+ * TVS_ERR_SIZE). */
+tvs_status tvs_band_cache_save(tvs_ctx *ctx, const char *state_dir,
+                               uint32_t sector);
+tvs_status tvs_band_cache_load(tvs_ctx *ctx, const char *state_dir,
+                               uint32_t sector);
+
+/* Message for the last failing I/O call that had no context. */
+const char *tvs_io_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -239,6 +239,7 @@ tvs_status tvs_create(const tvs_params *p, tvs_ctx **out) {
     if (e == hipSuccess) e = dev_alloc(&c->tab.dist, tab_n);
     if (e == hipSuccess) e = dev_alloc(&c->tab.off, tab_n);
     if (e == hipSuccess) e = dev_alloc(&c->tab.rcp, tab_n);
+    if (e == hipSuccess) e = dev_alloc(&c->tab.rcp_lo, tab_n);
     if (e == hipSuccess) e = dev_alloc(&c->tab.curv, tab_n);
     if (e == hipSuccess) e = dev_alloc(&c->tab.wt, tab_n);
     if (e == hipSuccess) e = dev_alloc(&c->tab.meta, tvs::kBandSlots);
@@ -266,6 +267,7 @@ tvs_status tvs_destroy(tvs_ctx *c) {
     dev_free(c->tab.dist);
     dev_free(c->tab.off);
     dev_free(c->tab.rcp);
+    dev_free(c->tab.rcp_lo);
     dev_free(c->tab.curv);
     dev_free(c->tab.wt);
     dev_free(c->tab.meta);

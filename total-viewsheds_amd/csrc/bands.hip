@@ -4,6 +4,7 @@
 // 32-bit total-order keys, the band template compile
 // (band_of_sight.rs:61-173) and the walk's per-step constant tables.
 #include "engine.hpp"
+#include "fastdiv.hpp"
 
 #pragma clang fp contract(off)
 
@@ -205,12 +206,6 @@ __global__ __launch_bounds__(kCompileThreads) void band_compile_kernel(
     }
 }
 
-// Steps whose distance keeps 1/d and every quotient in the normal range the
-// fast divide is proven for (with |dh| in [2^-40, 2^40] or 0).
-__device__ __forceinline__ bool regular_distance(float d) {
-    return d >= 0x1p-60f && d <= 0x1p60f;
-}
-
 // Per sector slot: cumulative offsets (block scan), 1/d, curvature term,
 // surface weight, and the SectorMeta ranges.
 __global__ __launch_bounds__(kScanThreads) void step_table_kernel(
@@ -244,7 +239,9 @@ __global__ __launch_bounds__(kScanThreads) void step_table_kernel(
         run += delta[i];
         const float d = dist[i];
         tab.off[tb + i] = run;
-        tab.rcp[tb + i] = 1.0f / d;
+        const Reciprocal y = split_reciprocal(d);
+        tab.rcp[tb + i] = y.hi;
+        tab.rcp_lo[tb + i] = y.lo;
         tab.curv[tb + i] = d / kEarthRadiusSquared;
         tab.wt[tb + i] = d * kTanOneRad;
         mn = run < mn ? run : mn;

@@ -48,6 +48,7 @@ struct StepTables {
     float *dist = nullptr;     // band_distances         (band_of_sight.rs:168)
     int32_t *off = nullptr;    // inclusive prefix sum of delta
     float *rcp = nullptr;      // RN(1 / dist)
+    float *rcp_lo = nullptr;   // RN((1 - dist * rcp) * rcp): 1/dist - rcp
     float *curv = nullptr;     // RN(dist / 12742000)    (kernel.rs:175)
     float *wt = nullptr;       // RN(dist * 0.0174533)   (kernel.rs:216)
     SectorMeta *meta = nullptr;

@@ -22,26 +22,13 @@
 #include <algorithm>
 
 #include "engine.hpp"
+#include "fastdiv.hpp"
 
 namespace tvs {
 namespace {
 
 constexpr uint32_t kBlock = 256;
 constexpr int kUnroll = 8;
-
-// RN(a / b) from y = RN(1 / b), for b > 0 normal and a with no
-// overflow/underflow of a/b or of the residual (checked on the host before
-// this path is chosen).  a*y is within 2 ulp of a/b; one residual correction
-// brings it within 1 ulp; the second correction is Markstein's theorem
-// (y within 1/2 ulp of 1/b and q within 1 ulp of a/b give
-// RN(q + (a - b q) y) = RN(a/b)).  Both residuals are exact FMAs.
-__device__ __forceinline__ float div_rn_recip(float a, float b, float y) {
-    const float q0 = a * y;
-    const float r0 = __builtin_fmaf(-q0, b, a);
-    const float q1 = __builtin_fmaf(r0, y, q0);
-    const float r1 = __builtin_fmaf(-q1, b, a);
-    return __builtin_fmaf(r1, y, q1);
-}
 
 __global__ __launch_bounds__(kBlock) void walk_plain_kernel(
     const float *__restrict__ elev, const int32_t *__restrict__ deltas,
@@ -136,6 +123,7 @@ struct BatchedArgs {
     const int32_t *off;
     const float *dist;
     const float *rcp;
+    const float *rcp_lo;
     const float *curv;
     const float *wt;
     const SectorMeta *meta;
@@ -192,6 +180,8 @@ __global__ __launch_bounds__(kBlock) void walk_batched_kernel(
         static_cast<const float *>(__builtin_assume_aligned(a.dist + tb, 256));
     const float *__restrict__ rcp =
         static_cast<const float *>(__builtin_assume_aligned(a.rcp + tb, 256));
+    const float *__restrict__ rcp_lo =
+        static_cast<const float *>(__builtin_assume_aligned(a.rcp_lo + tb, 256));
     const float *__restrict__ curv =
         static_cast<const float *>(__builtin_assume_aligned(a.curv + tb, 256));
     const float *__restrict__ wt =
@@ -234,7 +224,7 @@ __global__ __launch_bounds__(kBlock) void walk_batched_kernel(
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
             const float dh = h[u] - h0;
-            const float q = FAST ? div_rn_recip(dh, dist[k + u], rcp[k + u])
+            const float q = FAST ? div_by_reciprocal(dh, dist[k + u], rcp[k + u], rcp_lo[k + u])
                                  : dh / dist[k + u];
             const float angle = q - curv[k + u];
             const bool v = angle > max_angle;
@@ -249,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void walk_batched_kernel(
         const int32_t o = off[k];
         const float d = dist[k];
         const float dh = load_height<H>(rsrc, pov_bytes, soff(o)) - h0;
-        const float q = (FAST && k < n_fast) ? div_rn_recip(dh, d, rcp[k]) : dh / d;
+        const float q = (FAST && k < n_fast) ? div_by_reciprocal(dh, d, rcp[k], rcp_lo[k]) : dh / d;
         const float angle = q - curv[k];
         const bool v = angle > max_angle;
         if (CAPTURE && capture && v != visible) event(o);
@@ -354,6 +344,7 @@ hipError_t launch_walk_batched(tvs_ctx *c, uint32_t first_slot, uint32_t count,
     a.off = c->tab.off;
     a.dist = c->tab.dist;
     a.rcp = c->tab.rcp;
+    a.rcp_lo = c->tab.rcp_lo;
     a.curv = c->tab.curv;
     a.wt = c->tab.wt;
     a.meta = c->tab.meta;

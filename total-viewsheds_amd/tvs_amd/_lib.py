@@ -75,6 +75,24 @@ class Timing(ctypes.Structure):
     ]
 
 
+class BtHeader(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("data_size", ctypes.c_uint16),
+        ("is_float", ctypes.c_uint16),
+        ("horizontal_units", ctypes.c_uint16),
+        ("utm_zone", ctypes.c_uint16),
+        ("datum", ctypes.c_uint16),
+        ("projection_source", ctypes.c_uint16),
+        ("left", ctypes.c_double),
+        ("right", ctypes.c_double),
+        ("bottom", ctypes.c_double),
+        ("top", ctypes.c_double),
+        ("vertical_scale", ctypes.c_float),
+    ]
+
+
 class TvsError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"{message} (status {status})")
@@ -120,6 +138,15 @@ def load() -> ctypes.CDLL:
         "tvs_get_timing": (st, [vp, P(Timing)]),
         "tvs_last_error": (ctypes.c_char_p, [vp]),
         "tvs_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+        "tvs_bt_read_header": (st, [ctypes.c_char_p, P(BtHeader)]),
+        "tvs_bt_read_f32": (st, [ctypes.c_char_p, vp, ctypes.c_size_t]),
+        "tvs_bt_read_i16": (st, [ctypes.c_char_p, vp, ctypes.c_size_t]),
+        "tvs_bt_write": (st, [ctypes.c_char_p, P(BtHeader), vp]),
+        "tvs_bt_scale": (ctypes.c_double, [P(BtHeader)]),
+        "tvs_write_heatmap_png": (st, [ctypes.c_char_p, vp, ctypes.c_uint32, ctypes.c_uint32]),
+        "tvs_band_cache_save": (st, [vp, ctypes.c_char_p, ctypes.c_uint32]),
+        "tvs_band_cache_load": (st, [vp, ctypes.c_char_p, ctypes.c_uint32]),
+        "tvs_io_last_error": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -140,6 +167,12 @@ def _check(status: int, ctx=None) -> None:
     if status != OK:
         msg = load().tvs_last_error(ctx)
         raise TvsError(status, (msg or b"").decode() or
+                       load().tvs_status_string(status).decode())
+
+
+def _check_io(status: int) -> None:
+    if status != OK:
+        raise TvsError(status, (load().tvs_io_last_error() or b"").decode() or
                        load().tvs_status_string(status).decode())
 
 
@@ -279,6 +312,12 @@ class Engine:
         v = ctypes.c_uint64(0)
         self._c(load().tvs_ring_overflow_count(self._h, ctypes.byref(v)))
         return int(v.value)
+
+    def cache_save(self, state_dir: str, sector: int) -> None:
+        self._c(load().tvs_band_cache_save(self._h, state_dir.encode(), sector))
+
+    def cache_load(self, state_dir: str, sector: int) -> None:
+        self._c(load().tvs_band_cache_load(self._h, state_dir.encode(), sector))
 
     def sync(self) -> None:
         self._c(load().tvs_sync(self._h))
