@@ -176,6 +176,60 @@ tvs_status upload_common(tvs_ctx *c, bool integral, int64_t lo, int64_t hi) {
 
 }  // namespace
 
+namespace tvs {
+
+// Descriptor of this context's DEM for a batched launch over sector slots
+// [first, first + count); grows the band-surface buffer as needed.
+tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t capture,
+                        const WalkConfig &cfg, TileDesc *out) {
+    const uint32_t N = c->geo.computable_points_count;
+    if (!walk_shape_supported(cfg))
+        return fail(c, TVS_ERR_INVALID_ARGUMENT, "walk shape not compiled");
+    if (c->partial_cap < count) {
+        dev_free(c->partial);
+        c->partial_cap = 0;
+        TVS_HIP(c, dev_alloc(&c->partial, static_cast<size_t>(count) * 2ull * N),
+                "band surfaces");
+        c->partial_cap = count;
+    }
+    int32_t bias = 0;
+    for (uint32_t s = first; s < first + count; ++s)
+        bias = std::max(bias, std::max(c->meta_host[s].max_off, -c->meta_host[s].min_off));
+    TileDesc d{};
+    d.elev = c->has_i16 ? static_cast<const void *>(c->elev_i16)
+                        : static_cast<const void *>(c->elev_f32);
+    d.off = c->tab.off;
+    d.dist = c->tab.dist;
+    d.rcp = c->tab.rcp;
+    d.rcp_lo = c->tab.rcp_lo;
+    d.curv = c->tab.curv;
+    d.wt = c->tab.wt;
+    d.meta = c->tab.meta;
+    d.partial = c->partial;
+    d.cum = c->cum;
+    d.cum64 = c->cum64;
+    d.ring = capture ? c->ring : nullptr;
+    d.overflow = c->overflow;
+    d.stride = c->tab.stride;
+    d.n_steps = c->geo.band_deltas_size;
+    d.first_slot = first;
+    d.count = count;
+    d.N = N;
+    d.T = c->geo.tvs_width;
+    d.L = c->geo.max_los_as_points;
+    d.w = c->geo.dem_width;
+    d.obs = c->params.observer_height;
+    d.bias = bias;
+    d.blocks_per_pass = (N + cfg.block - 1) / cfg.block;
+    d.R = c->rings_per_band;
+    d.capture_slots = capture;
+    d.fast = c->fast_div_ok ? 1u : 0u;
+    *out = d;
+    return TVS_OK;
+}
+
+}  // namespace tvs
+
 extern "C" {
 
 const char *tvs_status_string(tvs_status st) {
@@ -233,6 +287,8 @@ tvs_status tvs_create(const tvs_params *p, tvs_ctx **out) {
     if (e == hipSuccess) e = dev_alloc(&c->cum, N);
     if (e == hipSuccess) e = dev_alloc(&c->cum64, N);
     if (e == hipSuccess) e = dev_alloc(&c->overflow, 1);
+    if (e == hipSuccess) e = dev_alloc(&c->d_tile, 1);
+    if (e == hipSuccess) e = dev_alloc(&c->d_begin, 4);
     c->tab.stride = std::max<uint32_t>(64u, (g.band_deltas_size + 63u) / 64u * 64u);
     const size_t tab_n = static_cast<size_t>(tvs::kBandSlots) * c->tab.stride;
     if (e == hipSuccess) e = dev_alloc(&c->tab.delta, tab_n);
@@ -276,6 +332,8 @@ tvs_status tvs_destroy(tvs_ctx *c) {
     dev_free(c->partial);
     dev_free(c->ring);
     dev_free(c->overflow);
+    dev_free(c->d_tile);
+    dev_free(c->d_begin);
     free_sort_scratch(c->sort);
     for (auto &ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -470,17 +528,23 @@ tvs_status tvs_run_sectors(tvs_ctx *c, uint32_t first, uint32_t count, tvs_metho
         TVS_HIP(c, hipEventRecord(c->ev[2], c->stream), "event");
         c->timing.walk_launches = count;
     } else {
-        const size_t need = static_cast<size_t>(count);
-        if (c->partial_cap < need) {
-            dev_free(c->partial);
-            c->partial_cap = 0;
-            TVS_HIP(c, dev_alloc(&c->partial, need * 2ull * N), "band surfaces");
-            c->partial_cap = need;
-        }
+        tvs::TileDesc d;
+        if ((st = tvs::prepare_tile(c, first, count, capture, c->walk_cfg, &d)) != TVS_OK)
+            return st;
+        const uint32_t n_items = d.blocks_per_pass * 2u * count;
+        const uint32_t begin[4] = {0u, n_items, 0u, N};
+        TVS_HIP(c, hipMemcpyAsync(c->d_tile, &d, sizeof d, hipMemcpyHostToDevice, c->stream),
+                "tile descriptor");
+        TVS_HIP(c, hipMemcpyAsync(c->d_begin, begin, sizeof begin, hipMemcpyHostToDevice,
+                                  c->stream),
+                "tile descriptor");
         TVS_HIP(c, hipEventRecord(c->ev[0], c->stream), "event");
-        TVS_HIP(c, tvs::launch_walk_batched(c, first, count, capture), "batched walk");
+        TVS_HIP(c, tvs::launch_walk_tiles(c->stream, c->d_tile, c->d_begin, 1, n_items,
+                                          c->has_i16, capture > 0, c->walk_cfg),
+                "batched walk");
         TVS_HIP(c, hipEventRecord(c->ev[1], c->stream), "event");
-        TVS_HIP(c, tvs::launch_reduce(c, count), "sector reduction");
+        TVS_HIP(c, tvs::launch_reduce_tiles(c->stream, c->d_tile, c->d_begin + 2, 1, N),
+                "sector reduction");
         TVS_HIP(c, hipEventRecord(c->ev[2], c->stream), "event");
         c->timing.walk_launches = 1;
     }

@@ -55,6 +55,44 @@ struct StepTables {
     uint32_t stride = 0;
 };
 
+// One DEM tile of a walk launch: where its heights, step tables and outputs
+// live and the geometry the walk needs.  A launch walks `count` sector slots
+// from `first_slot`, both directions, for all N points of every tile.
+struct TileDesc {
+    const void *elev;            // int16 or f32, w*w row-major
+    const int32_t *off;          // step tables, slot 0 (see StepTables)
+    const float *dist;
+    const float *rcp;
+    const float *rcp_lo;
+    const float *curv;
+    const float *wt;
+    const SectorMeta *meta;
+    float *partial;              // f32[count][2][N] band surfaces
+    float *cum;                  // f32[N] cumulative surfaces
+    double *cum64;               // f64[N]
+    uint32_t *ring;              // u32[capture_slots][2N][R] or null
+    unsigned long long *overflow;
+    uint32_t stride, n_steps, first_slot, count;
+    uint32_t N, T, L, w;
+    float obs;
+    int32_t bias;                // >= every |cumulative offset| of the slots
+    uint32_t blocks_per_pass;    // ceil(N / block)
+    uint32_t R, capture_slots;
+    uint32_t fast;               // heights passed the fast-divide range check
+};
+
+// Launch shape of the tile walk: points per work item (= threads per
+// block), steps unrolled per chunk, XCD-contiguous item mapping.
+struct WalkConfig {
+    uint32_t block = 256;
+    uint32_t unroll = 8;
+    bool xcd_contiguous = true;
+};
+
+// Compiled (block, unroll) shapes of the tile walk.
+#define TVS_WALK_SHAPES(X) \
+    X(256, 8) X(128, 8) X(64, 8) X(512, 8) X(256, 4) X(256, 16) X(128, 16)
+
 // Scratch for the device Axes::compute (keys, ids, radix-sort ping-pong,
 // histograms).
 struct SortScratch {
@@ -89,6 +127,9 @@ struct tvs_ctx {
     std::vector<uint8_t> band_ready;   // per slot: band data resident
     std::vector<tvs::SectorMeta> meta_host;
 
+    tvs::WalkConfig walk_cfg;
+    tvs::TileDesc *d_tile = nullptr;     // device copy of this DEM's descriptor
+    uint32_t *d_begin = nullptr;         // {0, items} and {0, N}
     float *cum = nullptr;      // f32[N]  cumulative surfaces
     double *cum64 = nullptr;   // f64[N]
     float *partial = nullptr;  // f32[sectors][2][N] (batched method)
@@ -112,12 +153,21 @@ hipError_t launch_step_tables(tvs_ctx *c, uint32_t first_slot, uint32_t count);
 
 // ---- launchers (walk.hip) ----
 hipError_t launch_walk_plain(tvs_ctx *c, uint32_t slot, uint32_t *ring_out);
-hipError_t launch_walk_batched(tvs_ctx *c, uint32_t first_slot,
-                               uint32_t count, uint32_t capture_slots);
-hipError_t launch_reduce(tvs_ctx *c, uint32_t count);
+bool walk_shape_supported(const WalkConfig &cfg);
+hipError_t launch_walk_tiles(hipStream_t s, const TileDesc *tiles,
+                             const uint32_t *item_begin, uint32_t n_tiles,
+                             uint32_t n_items, bool i16, bool capture,
+                             const WalkConfig &cfg);
+hipError_t launch_reduce_tiles(hipStream_t s, const TileDesc *tiles,
+                               const uint32_t *point_begin, uint32_t n_tiles,
+                               uint32_t n_points);
 hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, size_t n, uint32_t v);
 hipError_t launch_widen_i16(hipStream_t s, const int16_t *in, float *out,
                             size_t n);
+
+// ---- host side of a batched launch (api.cpp) ----
+tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t capture,
+                        const WalkConfig &cfg, TileDesc *out);
 
 // ---- host geometry (geometry.cpp) ----
 tvs_status dem_geometry(uint32_t width, float scale, uint32_t los,

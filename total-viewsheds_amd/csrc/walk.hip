@@ -9,16 +9,17 @@
 //    backward bands; the block then adds forward-then-backward into the
 //    cumulative surface, the order compute.rs:229 + kernel.rs:254 produce.
 //
-//  * walk_batched_kernel: every requested sector in one launch.  A work item
-//    is (sector, direction, 256 consecutive points); lanes are consecutive
-//    points of a row, so each step's gather is one contiguous row segment at
-//    a wave-uniform offset (scalar base + per-lane constant byte offset: no
-//    per-step address arithmetic in the vector unit).  The per-step constants
-//    (cumulative offset, distance, 1/distance, curvature term, surface
-//    weight) come from the sector's step table through the scalar cache.
-//    Each band's surface goes to a per-(sector, direction, point) slot; a
-//    reduction then adds them per point in sector order, forward before
-//    backward, giving the reference's f32 cumulative sum bit for bit.
+//  * walk_tiles_kernel: every requested sector of one or many DEM tiles in
+//    one launch.  A work item is (tile, sector, direction, BLOCK consecutive
+//    points); lanes are consecutive points of a row, so each step's gather is
+//    one contiguous row segment at a wave-uniform offset (scalar base +
+//    per-lane constant byte offset: no per-step address arithmetic in the
+//    vector unit).  The per-step constants (cumulative offset, distance,
+//    two-term reciprocal, curvature term, surface weight) come from the
+//    sector's step table through the scalar cache.  Each band's surface goes
+//    to a per-(sector, direction, point) slot; reduce_tiles_kernel then adds
+//    them per point in sector order, forward before backward, giving the
+//    reference's f32 cumulative sum bit for bit.
 #include <algorithm>
 
 #include "engine.hpp"
@@ -28,7 +29,6 @@ namespace tvs {
 namespace {
 
 constexpr uint32_t kBlock = 256;
-constexpr int kUnroll = 8;
 
 __global__ __launch_bounds__(kBlock) void walk_plain_kernel(
     const float *__restrict__ elev, const int32_t *__restrict__ deltas,
@@ -90,10 +90,10 @@ __global__ __launch_bounds__(kBlock) void walk_plain_kernel(
 }
 
 // Heights are read through a buffer resource whose base sits `bias`
-// elements below the DEM: the wave-uniform step offset (made non-negative
+// elements below the tile: the wave-uniform step offset (made non-negative
 // by the bias) is the scalar offset of the load and the lane's point is its
-// vector offset, so a step costs no vector address arithmetic.  The
-// resource's range covers exactly [elev - bias, elev + w*w).
+// vector offset.  The resource's range covers exactly [elev - bias,
+// elev + w*w).
 template <typename H>
 __device__ __forceinline__ float load_height(__amdgpu_buffer_rsrc_t rsrc,
                                              uint32_t pov_bytes,
@@ -119,75 +119,49 @@ __device__ __forceinline__ float load_height<float>(__amdgpu_buffer_rsrc_t rsrc,
 // Buffer resource word 3 for raw (untyped) loads on gfx9-family parts.
 constexpr int kBufferFlags = 0x00020000;
 
-struct BatchedArgs {
-    const int32_t *off;
-    const float *dist;
-    const float *rcp;
-    const float *rcp_lo;
-    const float *curv;
-    const float *wt;
-    const SectorMeta *meta;
-    uint32_t stride;
-    uint32_t n_steps;
-    uint32_t first_slot;
-    uint32_t N, T, L, w;
-    float obs;
-    int32_t bias;               // elements; >= every |offset| of the run
-    uint32_t blocks_per_pass;
-    uint32_t n_items;
-    uint32_t items_per_xcd;
-    float *partial;
-    uint32_t *ring;
-    uint32_t R;
-    uint32_t capture_slots;
-    unsigned long long *overflow;
-};
+// Pointers read out of a TileDesc are generic; naming them global lets the
+// wave-uniform table reads become scalar loads (s_load_dwordx8) instead of
+// per-lane loads plus a readfirstlane loop for the scalar offset.
+#define TVS_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const TVS_GLOBAL T *global_ptr(const T *p) {
+    return (const TVS_GLOBAL T *)p;
+}
+template <typename T>
+__device__ __forceinline__ TVS_GLOBAL T *global_ptr(T *p) {
+    return (TVS_GLOBAL T *)p;
+}
 
-template <typename H, bool FAST, bool CAPTURE>
-__global__ __launch_bounds__(kBlock) void walk_batched_kernel(
-    const H *__restrict__ elev, const BatchedArgs a) {
-    // Consecutive block ids land on the 8 XCDs round-robin; give each XCD a
-    // contiguous run of work items so its L2 holds one region of the DEM.
-    const uint32_t b = blockIdx.x;
-    const uint32_t item = (b & 7u) * a.items_per_xcd + (b >> 3);
-    if (item >= a.n_items) return;
-    const uint32_t pass = item / a.blocks_per_pass;      // 2*sector + dir
-    const uint32_t ob = item - pass * a.blocks_per_pass;
-    const uint32_t rel = pass >> 1;
-    const bool backward = (pass & 1u) != 0;
-    const uint32_t slot = a.first_slot + rel;
-
-    const uint32_t t_raw = ob * kBlock + threadIdx.x;
-    const bool active = t_raw < a.N;
-    const uint32_t t = active ? t_raw : a.N - 1u;       // spare lanes shadow
-    const uint32_t pov = (t / a.T + a.L) * a.w + (t % a.T + a.L);
+// One band per lane: (tile d, sector slot d.first_slot + rel, direction)
+// for point t.  FAST selects the split-reciprocal divide for the regular
+// steps (the tile's heights passed the host's range check).
+template <typename H, bool FAST, bool CAPTURE, int UNROLL>
+__device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool backward,
+                                          uint32_t t, bool active) {
+    const H *elev = static_cast<const H *>(d.elev);
+    const uint32_t slot = d.first_slot + rel;
+    const uint32_t pov = (t / d.T + d.L) * d.w + (t % d.T + d.L);
     const uint32_t pov_bytes = pov * static_cast<uint32_t>(sizeof(H));
-    const float h0 = static_cast<float>(elev[pov]) + a.obs;
+    const float h0 = static_cast<float>(global_ptr(elev)[pov]) + d.obs;
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<H *>(elev - a.bias), 0,
-        static_cast<int>((static_cast<uint32_t>(a.bias) + a.w * a.w) * sizeof(H)),
+        const_cast<H *>(elev - d.bias), 0,
+        static_cast<int>((static_cast<uint32_t>(d.bias) + d.w * d.w) * sizeof(H)),
         kBufferFlags);
     const int32_t sgn = backward ? -1 : 1;
     // Scalar byte offset of a step: (sgn * off + bias) * sizeof(H) >= 0.
     auto soff = [&](int32_t o) -> uint32_t {
-        return static_cast<uint32_t>(sgn * o + a.bias) * static_cast<uint32_t>(sizeof(H));
+        return static_cast<uint32_t>(sgn * o + d.bias) * static_cast<uint32_t>(sizeof(H));
     };
 
-    const size_t tb = static_cast<size_t>(slot) * a.stride;
-    const int32_t *__restrict__ off =
-        static_cast<const int32_t *>(__builtin_assume_aligned(a.off + tb, 256));
-    const float *__restrict__ dist =
-        static_cast<const float *>(__builtin_assume_aligned(a.dist + tb, 256));
-    const float *__restrict__ rcp =
-        static_cast<const float *>(__builtin_assume_aligned(a.rcp + tb, 256));
-    const float *__restrict__ rcp_lo =
-        static_cast<const float *>(__builtin_assume_aligned(a.rcp_lo + tb, 256));
-    const float *__restrict__ curv =
-        static_cast<const float *>(__builtin_assume_aligned(a.curv + tb, 256));
-    const float *__restrict__ wt =
-        static_cast<const float *>(__builtin_assume_aligned(a.wt + tb, 256));
-    const uint32_t n_fast = a.meta[slot].n_fast;
-    const uint32_t tail_start = a.meta[slot].tail_start;
+    const size_t tb = static_cast<size_t>(slot) * d.stride;
+    const TVS_GLOBAL int32_t *__restrict__ off = global_ptr(d.off + tb);
+    const TVS_GLOBAL float *__restrict__ dist = global_ptr(d.dist + tb);
+    const TVS_GLOBAL float *__restrict__ rcp = global_ptr(d.rcp + tb);
+    const TVS_GLOBAL float *__restrict__ rcp_lo = global_ptr(d.rcp_lo + tb);
+    const TVS_GLOBAL float *__restrict__ curv = global_ptr(d.curv + tb);
+    const TVS_GLOBAL float *__restrict__ wt = global_ptr(d.wt + tb);
+    const uint32_t n_fast = global_ptr(d.meta)[slot].n_fast;
+    const uint32_t tail_start = global_ptr(d.meta)[slot].tail_start;
 
     // `max_angle = v ? angle : max_angle` is f32::max(angle, max_angle)
     // (kernel.rs:233) for every input the walk meets: v is `angle >
@@ -199,30 +173,30 @@ __global__ __launch_bounds__(kBlock) void walk_batched_kernel(
     // Ring records (kernel.rs:141-243): an event is written exactly when the
     // visibility changes (opening and closing are exclusive), plus a final
     // close if the last step is visible.
-    const bool capture = CAPTURE && active && rel < a.capture_slots;
-    uint32_t *band_ring = nullptr;
+    const bool capture = CAPTURE && active && rel < d.capture_slots;
+    TVS_GLOBAL uint32_t *band_ring = nullptr;
     uint32_t ring_id = 1;
     unsigned dropped = 0;
     if (CAPTURE && capture) {
-        const uint32_t kernel_id = backward ? t + a.N : t;
-        band_ring = a.ring + static_cast<size_t>(rel) * 2u * a.N * a.R +
-                    static_cast<size_t>(kernel_id) * a.R;
-        if (1u < a.R) band_ring[1] = pov; else ++dropped;
+        const uint32_t kernel_id = backward ? t + d.N : t;
+        band_ring = global_ptr(d.ring) + static_cast<size_t>(rel) * 2u * d.N * d.R +
+                    static_cast<size_t>(kernel_id) * d.R;
+        if (1u < d.R) band_ring[1] = pov; else ++dropped;
     }
     auto event = [&](int32_t o) {
         ++ring_id;
-        if (ring_id < a.R) band_ring[ring_id] = pov + static_cast<uint32_t>(sgn * o);
+        if (ring_id < d.R) band_ring[ring_id] = pov + static_cast<uint32_t>(sgn * o);
         else ++dropped;
     };
 
     uint32_t k = 0;
-    const uint32_t fast_end = n_fast - n_fast % kUnroll;
-    for (; k < fast_end; k += kUnroll) {
-        float h[kUnroll];
+    const uint32_t fast_end = n_fast - n_fast % UNROLL;
+    for (; k < fast_end; k += UNROLL) {
+        float h[UNROLL];
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) h[u] = load_height<H>(rsrc, pov_bytes, soff(off[k + u]));
+        for (int u = 0; u < UNROLL; ++u) h[u] = load_height<H>(rsrc, pov_bytes, soff(off[k + u]));
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
+        for (int u = 0; u < UNROLL; ++u) {
             const float dh = h[u] - h0;
             const float q = FAST ? div_by_reciprocal(dh, dist[k + u], rcp[k + u], rcp_lo[k + u])
                                  : dh / dist[k + u];
@@ -237,9 +211,10 @@ __global__ __launch_bounds__(kBlock) void walk_batched_kernel(
     }
     for (; k < tail_start; ++k) {
         const int32_t o = off[k];
-        const float d = dist[k];
+        const float dk = dist[k];
         const float dh = load_height<H>(rsrc, pov_bytes, soff(o)) - h0;
-        const float q = (FAST && k < n_fast) ? div_by_reciprocal(dh, d, rcp[k], rcp_lo[k]) : dh / d;
+        const float q = (FAST && k < n_fast) ? div_by_reciprocal(dh, dk, rcp[k], rcp_lo[k])
+                                             : dh / dk;
         const float angle = q - curv[k];
         const bool v = angle > max_angle;
         if (CAPTURE && capture && v != visible) event(o);
@@ -248,14 +223,14 @@ __global__ __launch_bounds__(kBlock) void walk_batched_kernel(
         max_angle = v ? angle : max_angle;
         visible = v;
     }
-    if (k < a.n_steps) {
+    if (k < d.n_steps) {
         // Zero tail: delta 0 keeps the point, distance +0 gives the same
         // angle (dh/0 - 0: +inf, -inf or NaN) at every step, and the weight
         // 0 * 0.0174533 leaves the surface unchanged.
         const int32_t o = off[k];
         const float dh = load_height<H>(rsrc, pov_bytes, soff(o)) - h0;
         const float angle = dh / dist[k] - curv[k];
-        for (; k < a.n_steps; ++k) {
+        for (; k < d.n_steps; ++k) {
             const bool v = angle > max_angle;
             if (CAPTURE && capture && v != visible) event(o);
             max_angle = v ? angle : max_angle;
@@ -263,23 +238,66 @@ __global__ __launch_bounds__(kBlock) void walk_batched_kernel(
         }
     }
     if (CAPTURE && capture) {
-        if (visible) event(a.n_steps ? off[a.n_steps - 1] : 0);
-        if (a.R > 0) band_ring[0] = ring_id;
-        if (dropped) atomicAdd(a.overflow, static_cast<unsigned long long>(dropped));
+        if (visible) event(d.n_steps ? off[d.n_steps - 1] : 0);
+        if (d.R > 0) band_ring[0] = ring_id;
+        if (dropped) atomicAdd(d.overflow, static_cast<unsigned long long>(dropped));
     }
-    if (active) a.partial[static_cast<size_t>(pass) * a.N + t] = surface;
+    const uint32_t pass = 2u * rel + (backward ? 1u : 0u);
+    if (active) global_ptr(d.partial)[static_cast<size_t>(pass) * d.N + t] = surface;
+}
+
+// Index of the tile whose [begin[i], begin[i+1]) holds x (wave-uniform).
+__device__ __forceinline__ uint32_t find_tile(const uint32_t *__restrict__ begin,
+                                              uint32_t n_tiles, uint32_t x) {
+    uint32_t lo = 0, hi = n_tiles;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (begin[mid] <= x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <typename H, bool CAPTURE, int BLOCK, int UNROLL>
+__global__ __launch_bounds__(BLOCK) void walk_tiles_kernel(
+    const TileDesc *__restrict__ tiles, const uint32_t *__restrict__ item_begin,
+    uint32_t n_tiles, uint32_t n_items, uint32_t items_per_xcd) {
+    // With items_per_xcd != 0, consecutive block ids (dispatched to the 8
+    // XCDs round-robin) are remapped so each XCD takes a contiguous run of
+    // items: one DEM region per L2.
+    const uint32_t b = blockIdx.x;
+    const uint32_t item = items_per_xcd ? (b & 7u) * items_per_xcd + (b >> 3) : b;
+    if (item >= n_items) return;
+    const uint32_t ti = n_tiles > 1 ? find_tile(item_begin, n_tiles, item) : 0u;
+    const TileDesc d = tiles[ti];
+    const uint32_t local = item - item_begin[ti];
+    const uint32_t pass = local / d.blocks_per_pass;     // 2*sector + dir
+    const uint32_t ob = local - pass * d.blocks_per_pass;
+    const uint32_t t_raw = ob * BLOCK + threadIdx.x;
+    const bool active = t_raw < d.N;
+    const uint32_t t = active ? t_raw : d.N - 1u;       // spare lanes shadow
+    if (d.fast)
+        walk_band<H, true, CAPTURE, UNROLL>(d, pass >> 1, (pass & 1u) != 0, t, active);
+    else
+        walk_band<H, false, CAPTURE, UNROLL>(d, pass >> 1, (pass & 1u) != 0, t, active);
 }
 
 // Per point, in sector order: cum += forward; cum += backward (the f32
 // order of the reference's serial loop), and the same sums in f64.
-__global__ __launch_bounds__(kBlock) void reduce_kernel(
-    const float *__restrict__ partial, uint32_t count, uint32_t N,
-    float *__restrict__ cum, double *__restrict__ cum64) {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= N) return;
+__global__ __launch_bounds__(kBlock) void reduce_tiles_kernel(
+    const TileDesc *__restrict__ tiles, const uint32_t *__restrict__ point_begin,
+    uint32_t n_tiles, uint32_t n_points) {
+    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= n_points) return;
+    const uint32_t ti = n_tiles > 1 ? find_tile(point_begin, n_tiles, g) : 0u;
+    const TileDesc &d = tiles[ti];
+    const uint32_t N = d.N;
+    const uint32_t t = g - point_begin[ti];
+    const TVS_GLOBAL float *__restrict__ partial = global_ptr(d.partial);
+    TVS_GLOBAL float *cum = global_ptr(d.cum);
+    TVS_GLOBAL double *cum64 = global_ptr(d.cum64);
     float c = cum[t];
     double s = cum64[t];
-    for (uint32_t r = 0; r < count; ++r) {
+    for (uint32_t r = 0; r < d.count; ++r) {
         const float f = partial[static_cast<size_t>(2u * r) * N + t];
         const float b = partial[static_cast<size_t>(2u * r + 1u) * N + t];
         c = c + f;
@@ -303,23 +321,30 @@ __global__ void widen_i16_kernel(const int16_t *in, float *out, size_t n) {
         out[i] = static_cast<float>(in[i]);
 }
 
-template <typename H, bool FAST, bool CAPTURE>
-hipError_t launch_batched_t(tvs_ctx *c, const H *elev, const BatchedArgs &a,
-                            uint32_t grid) {
-    hipLaunchKernelGGL((walk_batched_kernel<H, FAST, CAPTURE>), dim3(grid),
-                       dim3(kBlock), 0, c->stream, elev, a);
+template <typename H, bool CAPTURE, int BLOCK, int UNROLL>
+hipError_t launch_tiles_t(hipStream_t s, const TileDesc *tiles, const uint32_t *item_begin,
+                          uint32_t n_tiles, uint32_t n_items, bool xcd_contiguous) {
+    const uint32_t per_xcd = xcd_contiguous ? (n_items + 7u) / 8u : 0u;
+    const uint32_t grid = xcd_contiguous ? per_xcd * 8u : n_items;
+    hipLaunchKernelGGL((walk_tiles_kernel<H, CAPTURE, BLOCK, UNROLL>), dim3(grid),
+                       dim3(BLOCK), 0, s, tiles, item_begin, n_tiles, n_items, per_xcd);
     return hipGetLastError();
 }
 
 template <typename H>
-hipError_t launch_batched_h(tvs_ctx *c, const H *elev, const BatchedArgs &a,
-                            uint32_t grid, bool fast, bool capture) {
-    if (fast) {
-        return capture ? launch_batched_t<H, true, true>(c, elev, a, grid)
-                       : launch_batched_t<H, true, false>(c, elev, a, grid);
-    }
-    return capture ? launch_batched_t<H, false, true>(c, elev, a, grid)
-                   : launch_batched_t<H, false, false>(c, elev, a, grid);
+hipError_t launch_tiles_h(hipStream_t s, const TileDesc *tiles, const uint32_t *item_begin,
+                          uint32_t n_tiles, uint32_t n_items, bool capture,
+                          const WalkConfig &cfg) {
+    if (capture)   // ring capture is a checking path: one shape
+        return launch_tiles_t<H, true, 256, 8>(s, tiles, item_begin, n_tiles, n_items,
+                                               cfg.xcd_contiguous);
+#define TVS_SHAPE(B, U)                                                                   \
+    if (cfg.block == B && cfg.unroll == U)                                                \
+        return launch_tiles_t<H, false, B, U>(s, tiles, item_begin, n_tiles, n_items,     \
+                                              cfg.xcd_contiguous);
+    TVS_WALK_SHAPES(TVS_SHAPE)
+#undef TVS_SHAPE
+    return hipErrorInvalidValue;
 }
 
 }  // namespace
@@ -337,48 +362,27 @@ hipError_t launch_walk_plain(tvs_ctx *c, uint32_t slot, uint32_t *ring_out) {
     return hipGetLastError();
 }
 
-hipError_t launch_walk_batched(tvs_ctx *c, uint32_t first_slot, uint32_t count,
-                               uint32_t capture_slots) {
-    const tvs_geometry &g = c->geo;
-    BatchedArgs a;
-    a.off = c->tab.off;
-    a.dist = c->tab.dist;
-    a.rcp = c->tab.rcp;
-    a.rcp_lo = c->tab.rcp_lo;
-    a.curv = c->tab.curv;
-    a.wt = c->tab.wt;
-    a.meta = c->tab.meta;
-    a.stride = c->tab.stride;
-    a.n_steps = g.band_deltas_size;
-    a.first_slot = first_slot;
-    a.N = g.computable_points_count;
-    a.T = g.tvs_width;
-    a.L = g.max_los_as_points;
-    a.w = g.dem_width;
-    a.obs = c->params.observer_height;
-    int32_t bias = 0;
-    for (uint32_t s = first_slot; s < first_slot + count; ++s)
-        bias = std::max(bias, std::max(c->meta_host[s].max_off, -c->meta_host[s].min_off));
-    a.bias = bias;
-    a.blocks_per_pass = (a.N + kBlock - 1) / kBlock;
-    a.n_items = a.blocks_per_pass * 2u * count;
-    a.items_per_xcd = (a.n_items + 7u) / 8u;
-    a.partial = c->partial;
-    a.ring = c->ring;
-    a.R = c->rings_per_band;
-    a.capture_slots = capture_slots;
-    a.overflow = c->overflow;
-    const uint32_t grid = a.items_per_xcd * 8u;
-    const bool capture = capture_slots > 0 && c->ring != nullptr;
-    if (c->has_i16)
-        return launch_batched_h<int16_t>(c, c->elev_i16, a, grid, c->fast_div_ok, capture);
-    return launch_batched_h<float>(c, c->elev_f32, a, grid, c->fast_div_ok, capture);
+bool walk_shape_supported(const WalkConfig &cfg) {
+#define TVS_SHAPE(B, U) if (cfg.block == B && cfg.unroll == U) return true;
+    TVS_WALK_SHAPES(TVS_SHAPE)
+#undef TVS_SHAPE
+    return false;
 }
 
-hipError_t launch_reduce(tvs_ctx *c, uint32_t count) {
-    const uint32_t N = c->geo.computable_points_count;
-    hipLaunchKernelGGL(reduce_kernel, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                       c->stream, c->partial, count, N, c->cum, c->cum64);
+hipError_t launch_walk_tiles(hipStream_t s, const TileDesc *tiles, const uint32_t *item_begin,
+                             uint32_t n_tiles, uint32_t n_items, bool i16, bool capture,
+                             const WalkConfig &cfg) {
+    if (n_items == 0) return hipSuccess;
+    return i16 ? launch_tiles_h<int16_t>(s, tiles, item_begin, n_tiles, n_items, capture, cfg)
+               : launch_tiles_h<float>(s, tiles, item_begin, n_tiles, n_items, capture, cfg);
+}
+
+hipError_t launch_reduce_tiles(hipStream_t s, const TileDesc *tiles,
+                               const uint32_t *point_begin, uint32_t n_tiles,
+                               uint32_t n_points) {
+    if (n_points == 0) return hipSuccess;
+    hipLaunchKernelGGL(reduce_tiles_kernel, dim3((n_points + kBlock - 1) / kBlock),
+                       dim3(kBlock), 0, s, tiles, point_begin, n_tiles, n_points);
     return hipGetLastError();
 }
 
