@@ -213,6 +213,68 @@ const char *tvs_last_error(const tvs_ctx *ctx);
 /* Static name of a status code. */
 const char *tvs_status_string(tvs_status status);
 
+/* ---- launch shape and autotuning of the batched walk --------------------- */
+
+/* Points per work item (threads per block), steps unrolled per chunk, and
+ * whether consecutive work items stay on one XCD; `ms` is a timing result.
+ * No reference counterpart: the Vulkan path's fixed 8x8x4 workgroup
+ * (lib.rs:23, gpu.rs:10) is the only shape the reference has. */
+typedef struct tvs_walk_shape {
+    uint32_t block;
+    uint32_t unroll;
+    uint32_t xcd_contiguous;
+    float ms;
+} tvs_walk_shape;
+
+tvs_status tvs_set_walk_shape(tvs_ctx *ctx, const tvs_walk_shape *shape);
+tvs_status tvs_get_walk_shape(const tvs_ctx *ctx, tvs_walk_shape *out);
+
+/* Number of shapes compiled into the library (x2 item mappings). */
+size_t tvs_walk_shape_count(void);
+
+/* Time every compiled shape on the context's own DEM over sectors
+ * [first, first+count) (`reps` timed runs after one warm-up each), keep the
+ * fastest as the context's shape, return all timings in `results` (up to
+ * `max_results`) and the winner in `best`.  Cumulative surfaces are
+ * preserved. */
+tvs_status tvs_autotune(tvs_ctx *ctx, uint32_t first_sector, uint32_t sector_count,
+                        uint32_t reps, tvs_walk_shape *results, size_t max_results,
+                        size_t *n_results, tvs_walk_shape *best);
+
+/* ---- many DEM tiles in one launch ------------------------------------------ */
+
+/* A batch of independent tiles (each a `Compute::new` of its own,
+ * compute.rs:27-84, with its own width, spacing and line of sight) whose
+ * 180 sectors all run in one walk launch and one reduction launch, replayed
+ * from a captured hipGraph.  Tiles of equal (width, band size) share band
+ * data (band_of_sight.rs:61-173 reads no elevation). */
+typedef struct tvs_batch tvs_batch;
+
+tvs_status tvs_batch_create(const tvs_params *tiles, uint32_t n_tiles, tvs_batch **out);
+tvs_status tvs_batch_destroy(tvs_batch *batch);
+tvs_status tvs_batch_tile_geometry(const tvs_batch *batch, uint32_t tile,
+                                   tvs_geometry *out);
+tvs_status tvs_batch_upload_i16(tvs_batch *batch, uint32_t tile,
+                                const int16_t *elevations, size_t count);
+tvs_status tvs_batch_upload_f32(tvs_batch *batch, uint32_t tile,
+                                const float *elevations, size_t count);
+/* Band data for all 180 sectors of every distinct (width, band size). */
+tvs_status tvs_batch_compute_bands(tvs_batch *batch);
+tvs_status tvs_batch_set_walk_shape(tvs_batch *batch, const tvs_walk_shape *shape);
+/* All tiles x 180 sectors; `use_graph` replays the captured launch pair.
+ * Asynchronous; surfaces accumulate as in tvs_run_sectors. */
+tvs_status tvs_batch_run(tvs_batch *batch, uint32_t use_graph);
+tvs_status tvs_batch_sync(tvs_batch *batch);
+tvs_status tvs_batch_reset_surfaces(tvs_batch *batch);
+tvs_status tvs_batch_read_surfaces(tvs_batch *batch, uint32_t tile, float *out,
+                                   size_t n);
+tvs_status tvs_batch_read_surfaces_f64(tvs_batch *batch, uint32_t tile, double *out,
+                                       size_t n);
+/* walk_ms = walk (or whole graph) time, reduce_ms = reduction time (0 when
+ * the graph ran); point_steps over all tiles. */
+tvs_status tvs_batch_get_timing(tvs_batch *batch, tvs_timing *out);
+const char *tvs_batch_last_error(const tvs_batch *batch);
+
 /* ---- host I/O around the walk ------------------------------------------ */
 
 /* `.bt` header fields (input.rs:27-70, read at input.rs:90-113). */

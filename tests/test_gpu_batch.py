@@ -1,0 +1,95 @@
+"""Many tiles in one launch (tvs_batch), hipGraph replay, and the launch
+shape tuner: every tile's surfaces equal its own single-context run and the
+oracle, bit for bit."""
+import os
+
+import numpy as np
+import pytest
+
+import tvs_amd
+from tvs_amd import demgen
+
+pytestmark = pytest.mark.gpu
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def small_tiles():
+    """Six uneven tiles (two share width and band size with others)."""
+    specs = [(129, 30.0, 1), (87, 90.0, 2), (129, 30.0, 3), (101, 30.0, 4), (87, 90.0, 5),
+             (61, 90.0, 6)]
+    out = []
+    for w, sp, seed in specs:
+        los = int((w // 3) * sp)
+        out.append(demgen.Workload(f"t{seed}", w, sp, los,
+                                   demgen.fractal_dem(w, sp, 2.4, 4000.0, seed).reshape(-1)))
+    return out
+
+
+@pytest.fixture(scope="module")
+def tiles_and_truth(oracle):
+    tiles = small_tiles()
+    truth = []
+    for t in tiles:
+        dem = oracle.dem_new(t.width, t.spacing, t.max_line_of_sight)
+        cum, _, _ = oracle.compute_run(dem, t.elevations.astype(np.float32), t.observer_height,
+                                       t.rings_per_km, rings=False, n_threads=THREADS)
+        truth.append(cum)
+    return tiles, truth
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_matches_oracle(tiles_and_truth, use_graph):
+    tiles, truth = tiles_and_truth
+    print(f"batch of {len(tiles)} tiles, graph={use_graph}", flush=True)
+    with tvs_amd.Batch([(t.width, t.spacing, t.max_line_of_sight, t.rings_per_km,
+                         t.observer_height) for t in tiles]) as b:
+        for i, t in enumerate(tiles):
+            b.upload(i, t.elevations)
+        b.compute_bands()
+        b.run(use_graph)
+        for i in range(len(tiles)):
+            np.testing.assert_array_equal(b.surfaces(i), truth[i], err_msg=f"tile {i}")
+        # Replays: eager reset between them, every replay equal to the first.
+        for rep in range(3):
+            b.reset_surfaces()
+            b.run(use_graph)
+            b.sync()
+            for i in range(len(tiles)):
+                np.testing.assert_array_equal(b.surfaces(i), truth[i],
+                                              err_msg=f"replay {rep} tile {i}")
+
+
+def test_batch_shapes_agree(tiles_and_truth):
+    tiles, truth = tiles_and_truth
+    with tvs_amd.Batch([(t.width, t.spacing, t.max_line_of_sight, t.rings_per_km,
+                         t.observer_height) for t in tiles]) as b:
+        for i, t in enumerate(tiles):
+            b.upload(i, t.elevations)
+        b.compute_bands()
+        for block, unroll, xcd in [(64, 8, False), (128, 16, True), (512, 8, False),
+                                   (256, 4, True)]:
+            print(f"batch shape {block}x{unroll} xcd={xcd}", flush=True)
+            b.set_walk_shape(block, unroll, xcd)
+            b.reset_surfaces()
+            b.run(True)
+            for i in range(len(tiles)):
+                np.testing.assert_array_equal(b.surfaces(i), truth[i])
+
+
+def test_autotune_keeps_results_exact(tiles_and_truth):
+    tiles, truth = tiles_and_truth
+    t = tiles[0]
+    with tvs_amd.Engine(t.width, t.spacing, t.max_line_of_sight, t.rings_per_km,
+                        t.observer_height) as e:
+        e.upload(t.elevations)
+        e.compute_bands(0, 180)
+        e.run(0, 180, "batched")
+        before = e.surfaces()
+        best, table = e.autotune(0, 30, reps=1)
+        print("autotune", best, flush=True)
+        assert len(table) == 2 * 7 and best["ms"] > 0
+        np.testing.assert_array_equal(e.surfaces(), before)   # surfaces preserved
+        assert e.walk_shape()["block"] == best["block"]
+        e.reset_surfaces()
+        e.run(0, 180, "batched")
+        np.testing.assert_array_equal(e.surfaces(), truth[0])

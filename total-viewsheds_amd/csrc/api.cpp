@@ -144,15 +144,7 @@ tvs_status read_meta(tvs_ctx *c, uint32_t first, uint32_t count) {
     TVS_HIP(c, hipMemcpy(c->meta_host.data() + first, c->tab.meta + first,
                          count * sizeof(tvs::SectorMeta), hipMemcpyDeviceToHost),
             "band table metadata");
-    for (uint32_t s = first; s < first + count; ++s) {
-        const tvs_status st = check_bounds(c, s);
-        if (st != TVS_OK) {
-            c->band_ready[s] = 0;
-            return st;
-        }
-        c->band_ready[s] = 1;
-    }
-    return TVS_OK;
+    return tvs::validate_slots(c, first, count);
 }
 
 tvs_status ensure_ring(tvs_ctx *c, size_t slots) {
@@ -177,6 +169,20 @@ tvs_status upload_common(tvs_ctx *c, bool integral, int64_t lo, int64_t hi) {
 }  // namespace
 
 namespace tvs {
+
+// Bounds check of slots [first, first+count) against this context's
+// geometry (meta_host already filled); marks them ready.
+tvs_status validate_slots(tvs_ctx *c, uint32_t first, uint32_t count) {
+    for (uint32_t s = first; s < first + count; ++s) {
+        const tvs_status st = check_bounds(c, s);
+        if (st != TVS_OK) {
+            c->band_ready[s] = 0;
+            return st;
+        }
+        c->band_ready[s] = 1;
+    }
+    return TVS_OK;
+}
 
 // Descriptor of this context's DEM for a batched launch over sector slots
 // [first, first + count); grows the band-surface buffer as needed.

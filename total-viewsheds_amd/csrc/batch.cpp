@@ -1,0 +1,338 @@
+// batch.cpp -- many DEM tiles, all 180 sectors, one device launch.
+//
+// A batch owns one context per tile (geometry, heights, surfaces).  Band
+// data depends only on (width, band size) (band_of_sight.rs:61-173 reads no
+// elevation), so tiles of equal (w, B) share the first such tile's step
+// tables.  A run is one tile-walk launch over every (tile, sector,
+// direction, point block) item, largest bands first, and one reduction
+// launch over every point of every tile; the pair is captured once in a
+// hipGraph and replayed.
+#include <algorithm>
+#include <map>
+#include <new>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace tvs {
+tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t capture,
+                        const WalkConfig &cfg, TileDesc *out);
+}
+
+struct tvs_batch {
+    std::vector<tvs_ctx *> tiles;
+    std::vector<uint32_t> owner;     // tile whose band tables a tile uses
+    std::vector<uint32_t> order;     // launch order (largest per-item work first)
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[3] = {};
+    tvs::TileDesc *d_tiles = nullptr;
+    uint32_t *d_item_begin = nullptr;
+    uint32_t *d_point_begin = nullptr;
+    uint32_t n_items = 0, n_points = 0;
+    tvs::WalkConfig cfg;
+    bool prepared = false;
+    bool i16 = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    tvs_timing timing{};
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_batch_error;
+
+tvs_status bfail(tvs_batch *b, tvs_status st, const std::string &msg) {
+    (b ? b->err : g_batch_error) = msg;
+    return st;
+}
+
+#define TVS_BHIP(b, expr, what)                                                         \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess)                                                           \
+            return bfail((b), _e == hipErrorOutOfMemory ? TVS_ERR_OUT_OF_MEMORY         \
+                                                        : TVS_ERR_HIP,                  \
+                         std::string(what) + ": " + hipGetErrorString(_e));            \
+    } while (0)
+
+tvs_status from_tile(tvs_batch *b, uint32_t i, tvs_status st) {
+    if (st != TVS_OK)
+        b->err = "tile " + std::to_string(i) + ": " + tvs_last_error(b->tiles[i]);
+    return st;
+}
+
+void drop_graph(tvs_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->prepared = false;
+}
+
+// Upload the descriptors and item/point prefixes for the current shape.
+tvs_status prepare(tvs_batch *b) {
+    if (b->prepared) return TVS_OK;
+    const size_t n = b->tiles.size();
+    bool all_i16 = true, any_i16 = false;
+    for (tvs_ctx *c : b->tiles) {
+        if (!c->has_f32) return bfail(b, TVS_ERR_STATE, "a tile has no elevations");
+        all_i16 = all_i16 && c->has_i16;
+        any_i16 = any_i16 || c->has_i16;
+    }
+    if (any_i16 && !all_i16)
+        return bfail(b, TVS_ERR_STATE, "tiles mix int16 and f32 heights");
+    b->i16 = all_i16;
+    std::vector<tvs::TileDesc> descs(n);
+    std::vector<uint32_t> item_begin(n + 1, 0), point_begin(n + 1, 0);
+    for (size_t k = 0; k < n; ++k) {
+        const uint32_t i = b->order[k];
+        tvs_ctx *c = b->tiles[i];
+        tvs_ctx *o = b->tiles[b->owner[i]];
+        for (uint32_t s = 0; s < tvs::kSectorSteps; ++s)
+            if (!c->band_ready[s]) return bfail(b, TVS_ERR_STATE, "tile " + std::to_string(i) + " has no band data");
+        tvs::TileDesc d;
+        tvs_status st = tvs::prepare_tile(c, 0, tvs::kSectorSteps, 0, b->cfg, &d);
+        if (st != TVS_OK) return from_tile(b, i, st);
+        d.off = o->tab.off;
+        d.dist = o->tab.dist;
+        d.rcp = o->tab.rcp;
+        d.rcp_lo = o->tab.rcp_lo;
+        d.curv = o->tab.curv;
+        d.wt = o->tab.wt;
+        d.meta = o->tab.meta;
+        d.stride = o->tab.stride;
+        descs[k] = d;
+        item_begin[k + 1] = item_begin[k] + d.blocks_per_pass * 2u * tvs::kSectorSteps;
+        point_begin[k + 1] = point_begin[k] + d.N;
+    }
+    b->n_items = item_begin[n];
+    b->n_points = point_begin[n];
+    (void)hipFree(b->d_tiles);
+    (void)hipFree(b->d_item_begin);
+    (void)hipFree(b->d_point_begin);
+    b->d_tiles = nullptr;
+    b->d_item_begin = b->d_point_begin = nullptr;
+    TVS_BHIP(b, hipMalloc(reinterpret_cast<void **>(&b->d_tiles), n * sizeof(tvs::TileDesc)), "descriptors");
+    TVS_BHIP(b, hipMalloc(reinterpret_cast<void **>(&b->d_item_begin), (n + 1) * sizeof(uint32_t)), "descriptors");
+    TVS_BHIP(b, hipMalloc(reinterpret_cast<void **>(&b->d_point_begin), (n + 1) * sizeof(uint32_t)), "descriptors");
+    TVS_BHIP(b, hipMemcpy(b->d_tiles, descs.data(), n * sizeof(tvs::TileDesc), hipMemcpyHostToDevice), "descriptors");
+    TVS_BHIP(b, hipMemcpy(b->d_item_begin, item_begin.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice), "descriptors");
+    TVS_BHIP(b, hipMemcpy(b->d_point_begin, point_begin.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice), "descriptors");
+    b->prepared = true;
+    return TVS_OK;
+}
+
+tvs_status enqueue(tvs_batch *b) {
+    TVS_BHIP(b, tvs::launch_walk_tiles(b->stream, b->d_tiles, b->d_item_begin,
+                                       static_cast<uint32_t>(b->tiles.size()), b->n_items,
+                                       b->i16, false, b->cfg),
+             "tile walk");
+    TVS_BHIP(b, tvs::launch_reduce_tiles(b->stream, b->d_tiles, b->d_point_begin,
+                                         static_cast<uint32_t>(b->tiles.size()), b->n_points),
+             "tile reduction");
+    return TVS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *tvs_batch_last_error(const tvs_batch *b) {
+    return b ? b->err.c_str() : g_batch_error.c_str();
+}
+
+tvs_status tvs_batch_create(const tvs_params *tiles, uint32_t n_tiles, tvs_batch **out) {
+    if (!tiles || !out || n_tiles == 0)
+        return bfail(nullptr, TVS_ERR_INVALID_ARGUMENT, "no tiles");
+    *out = nullptr;
+    tvs_batch *b = new (std::nothrow) tvs_batch();
+    if (!b) return bfail(nullptr, TVS_ERR_OUT_OF_MEMORY, "host allocation");
+    std::map<std::pair<uint32_t, uint32_t>, uint32_t> first_of;
+    for (uint32_t i = 0; i < n_tiles; ++i) {
+        tvs_params p = tiles[i];
+        p.ring_capture_sectors = 0;
+        tvs_ctx *c = nullptr;
+        const tvs_status st = tvs_create(&p, &c);
+        if (st != TVS_OK) {
+            const std::string msg = "tile " + std::to_string(i) + ": " + tvs_last_error(nullptr);
+            tvs_batch_destroy(b);
+            return bfail(nullptr, st, msg);
+        }
+        b->tiles.push_back(c);
+        const auto key = std::make_pair(c->geo.dem_width, c->geo.band_size);
+        auto it = first_of.find(key);
+        b->owner.push_back(it == first_of.end() ? i : it->second);
+        if (it == first_of.end()) first_of[key] = i;
+    }
+    // Largest per-item work (steps per band) first, so the long items start
+    // early and short ones fill the end of the launch.
+    b->order.resize(n_tiles);
+    std::iota(b->order.begin(), b->order.end(), 0u);
+    std::stable_sort(b->order.begin(), b->order.end(), [&](uint32_t x, uint32_t y) {
+        return b->tiles[x]->geo.band_deltas_size > b->tiles[y]->geo.band_deltas_size;
+    });
+    b->cfg.xcd_contiguous = false;   // mixed item costs: keep the XCDs balanced
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipEventCreate(&b->ev[k]);
+    if (e != hipSuccess) {
+        tvs_batch_destroy(b);
+        return bfail(nullptr, TVS_ERR_HIP, std::string("stream: ") + hipGetErrorString(e));
+    }
+    *out = b;
+    return TVS_OK;
+}
+
+tvs_status tvs_batch_destroy(tvs_batch *b) {
+    if (!b) return TVS_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    for (tvs_ctx *c : b->tiles) tvs_destroy(c);
+    if (b->d_tiles) (void)hipFree(b->d_tiles);
+    if (b->d_item_begin) (void)hipFree(b->d_item_begin);
+    if (b->d_point_begin) (void)hipFree(b->d_point_begin);
+    for (auto &ev : b->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return TVS_OK;
+}
+
+tvs_status tvs_batch_tile_geometry(const tvs_batch *b, uint32_t tile, tvs_geometry *out) {
+    if (!b || !out || tile >= b->tiles.size())
+        return bfail(nullptr, TVS_ERR_INVALID_ARGUMENT, "tile index");
+    *out = b->tiles[tile]->geo;
+    return TVS_OK;
+}
+
+tvs_status tvs_batch_upload_i16(tvs_batch *b, uint32_t tile, const int16_t *elevations,
+                                size_t count) {
+    if (!b || tile >= b->tiles.size()) return bfail(b, TVS_ERR_INVALID_ARGUMENT, "tile index");
+    drop_graph(b);
+    return from_tile(b, tile, tvs_upload_dem_i16(b->tiles[tile], elevations, count));
+}
+
+tvs_status tvs_batch_upload_f32(tvs_batch *b, uint32_t tile, const float *elevations,
+                                size_t count) {
+    if (!b || tile >= b->tiles.size()) return bfail(b, TVS_ERR_INVALID_ARGUMENT, "tile index");
+    drop_graph(b);
+    return from_tile(b, tile, tvs_upload_dem_f32(b->tiles[tile], elevations, count));
+}
+
+tvs_status tvs_batch_compute_bands(tvs_batch *b) {
+    if (!b) return bfail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null batch");
+    float ms = 0.0f;
+    for (uint32_t i = 0; i < b->tiles.size(); ++i) {
+        if (b->owner[i] != i) continue;
+        const tvs_status st = tvs_compute_bands(b->tiles[i], 0, tvs::kSectorSteps);
+        if (st != TVS_OK) return from_tile(b, i, st);
+        ms += b->tiles[i]->timing.bands_ms;
+    }
+    for (uint32_t i = 0; i < b->tiles.size(); ++i) {
+        const uint32_t o = b->owner[i];
+        if (o == i) continue;
+        tvs_ctx *c = b->tiles[i];
+        c->meta_host = b->tiles[o]->meta_host;
+        const tvs_status st = tvs::validate_slots(c, 0, tvs::kSectorSteps);
+        if (st != TVS_OK) return from_tile(b, i, st);
+    }
+    drop_graph(b);
+    b->timing.bands_ms = ms;
+    return TVS_OK;
+}
+
+tvs_status tvs_batch_set_walk_shape(tvs_batch *b, const tvs_walk_shape *shape) {
+    if (!b || !shape) return bfail(b, TVS_ERR_INVALID_ARGUMENT, "null argument");
+    tvs::WalkConfig cfg{shape->block, shape->unroll, shape->xcd_contiguous != 0};
+    if (!tvs::walk_shape_supported(cfg))
+        return bfail(b, TVS_ERR_INVALID_ARGUMENT, "walk shape not compiled");
+    b->cfg = cfg;
+    drop_graph(b);
+    return TVS_OK;
+}
+
+tvs_status tvs_batch_run(tvs_batch *b, uint32_t use_graph) {
+    if (!b) return bfail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null batch");
+    tvs_status st = prepare(b);
+    if (st != TVS_OK) return st;
+    uint64_t steps = 0;
+    for (tvs_ctx *c : b->tiles)
+        steps += 180ull * 2ull * c->geo.computable_points_count * c->geo.band_deltas_size;
+    b->timing = tvs_timing{0.0f, 0.0f, b->timing.bands_ms, 1u, steps};
+    if (use_graph) {
+        if (!b->exec) {
+            TVS_BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "capture");
+            const tvs_status es = enqueue(b);
+            hipGraph_t g = nullptr;
+            const hipError_t ce = hipStreamEndCapture(b->stream, &g);
+            if (es != TVS_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return es;
+            }
+            TVS_BHIP(b, ce, "capture");
+            b->graph = g;
+            TVS_BHIP(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph");
+        }
+        TVS_BHIP(b, hipEventRecord(b->ev[0], b->stream), "event");
+        TVS_BHIP(b, hipGraphLaunch(b->exec, b->stream), "graph launch");
+        TVS_BHIP(b, hipEventRecord(b->ev[1], b->stream), "event");
+        TVS_BHIP(b, hipEventRecord(b->ev[2], b->stream), "event");
+        return TVS_OK;
+    }
+    TVS_BHIP(b, hipEventRecord(b->ev[0], b->stream), "event");
+    TVS_BHIP(b, tvs::launch_walk_tiles(b->stream, b->d_tiles, b->d_item_begin,
+                                       static_cast<uint32_t>(b->tiles.size()), b->n_items,
+                                       b->i16, false, b->cfg),
+             "tile walk");
+    TVS_BHIP(b, hipEventRecord(b->ev[1], b->stream), "event");
+    TVS_BHIP(b, tvs::launch_reduce_tiles(b->stream, b->d_tiles, b->d_point_begin,
+                                         static_cast<uint32_t>(b->tiles.size()), b->n_points),
+             "tile reduction");
+    TVS_BHIP(b, hipEventRecord(b->ev[2], b->stream), "event");
+    return TVS_OK;
+}
+
+tvs_status tvs_batch_sync(tvs_batch *b) {
+    if (!b) return bfail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null batch");
+    TVS_BHIP(b, hipStreamSynchronize(b->stream), "sync");
+    return TVS_OK;
+}
+
+tvs_status tvs_batch_reset_surfaces(tvs_batch *b) {
+    if (!b) return bfail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null batch");
+    TVS_BHIP(b, hipStreamSynchronize(b->stream), "sync");
+    for (uint32_t i = 0; i < b->tiles.size(); ++i) {
+        tvs_status st = tvs_reset_surfaces(b->tiles[i]);
+        if (st == TVS_OK) st = tvs_sync(b->tiles[i]);
+        if (st != TVS_OK) return from_tile(b, i, st);
+    }
+    return TVS_OK;
+}
+
+tvs_status tvs_batch_read_surfaces(tvs_batch *b, uint32_t tile, float *out, size_t n) {
+    if (!b || tile >= b->tiles.size()) return bfail(b, TVS_ERR_INVALID_ARGUMENT, "tile index");
+    TVS_BHIP(b, hipStreamSynchronize(b->stream), "sync");
+    return from_tile(b, tile, tvs_read_surfaces(b->tiles[tile], out, n));
+}
+
+tvs_status tvs_batch_read_surfaces_f64(tvs_batch *b, uint32_t tile, double *out, size_t n) {
+    if (!b || tile >= b->tiles.size()) return bfail(b, TVS_ERR_INVALID_ARGUMENT, "tile index");
+    TVS_BHIP(b, hipStreamSynchronize(b->stream), "sync");
+    return from_tile(b, tile, tvs_read_surfaces_f64(b->tiles[tile], out, n));
+}
+
+tvs_status tvs_batch_get_timing(tvs_batch *b, tvs_timing *out) {
+    if (!b || !out) return bfail(b, TVS_ERR_INVALID_ARGUMENT, "null argument");
+    TVS_BHIP(b, hipEventSynchronize(b->ev[2]), "event");
+    float ms = 0.0f;
+    TVS_BHIP(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]), "event time");
+    b->timing.walk_ms = ms;
+    TVS_BHIP(b, hipEventElapsedTime(&ms, b->ev[1], b->ev[2]), "event time");
+    b->timing.reduce_ms = ms;
+    *out = b->timing;
+    return TVS_OK;
+}
+
+}  // extern "C"

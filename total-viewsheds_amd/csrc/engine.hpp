@@ -166,6 +166,7 @@ hipError_t launch_widen_i16(hipStream_t s, const int16_t *in, float *out,
                             size_t n);
 
 // ---- host side of a batched launch (api.cpp) ----
+tvs_status validate_slots(tvs_ctx *c, uint32_t first, uint32_t count);
 tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t capture,
                         const WalkConfig &cfg, TileDesc *out);
 

@@ -1,0 +1,117 @@
+// tune.cpp -- launch-shape selection for the batched walk.
+//
+// The shapes compiled into the library (points per work item = threads per
+// block, steps unrolled per chunk, XCD-contiguous or round-robin item
+// mapping) are timed on the context's own DEM and band data; the fastest
+// becomes the context's shape.  The cumulative surfaces are saved and
+// restored around the timing runs.
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace {
+
+std::vector<tvs::WalkConfig> compiled_shapes() {
+    std::vector<tvs::WalkConfig> out;
+#define TVS_SHAPE(B, U)                          \
+    out.push_back(tvs::WalkConfig{B, U, true});  \
+    out.push_back(tvs::WalkConfig{B, U, false});
+    TVS_WALK_SHAPES(TVS_SHAPE)
+#undef TVS_SHAPE
+    return out;
+}
+
+tvs_status tfail(tvs_ctx *c, tvs_status st, const std::string &msg) {
+    c->err = msg;
+    return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+tvs_status tvs_set_walk_shape(tvs_ctx *c, const tvs_walk_shape *shape) {
+    if (!c || !shape) return TVS_ERR_INVALID_ARGUMENT;
+    tvs::WalkConfig cfg{shape->block, shape->unroll, shape->xcd_contiguous != 0};
+    if (!tvs::walk_shape_supported(cfg))
+        return tfail(c, TVS_ERR_INVALID_ARGUMENT, "walk shape not compiled");
+    c->walk_cfg = cfg;
+    return TVS_OK;
+}
+
+tvs_status tvs_get_walk_shape(const tvs_ctx *c, tvs_walk_shape *out) {
+    if (!c || !out) return TVS_ERR_INVALID_ARGUMENT;
+    out->block = c->walk_cfg.block;
+    out->unroll = c->walk_cfg.unroll;
+    out->xcd_contiguous = c->walk_cfg.xcd_contiguous ? 1u : 0u;
+    out->ms = 0.0f;
+    return TVS_OK;
+}
+
+size_t tvs_walk_shape_count(void) { return compiled_shapes().size(); }
+
+tvs_status tvs_autotune(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t reps,
+                        tvs_walk_shape *results, size_t max_results, size_t *n_results,
+                        tvs_walk_shape *best) {
+    if (!c) return TVS_ERR_INVALID_ARGUMENT;
+    if (reps == 0) reps = 1;
+    const uint32_t N = c->geo.computable_points_count;
+    if (N == 0) return tfail(c, TVS_ERR_STATE, "no computable points");
+    // Keep the caller's cumulative surfaces.
+    float *saved = nullptr;
+    double *saved64 = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&saved), N * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&saved64), N * sizeof(double));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(saved, c->cum, N * sizeof(float), hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(saved64, c->cum64, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(saved);
+        (void)hipFree(saved64);
+        return tfail(c, TVS_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+    }
+    const tvs::WalkConfig keep = c->walk_cfg;
+    tvs::WalkConfig winner = keep;
+    float winner_ms = 1e30f;
+    size_t k = 0;
+    tvs_status st = TVS_OK;
+    for (const tvs::WalkConfig &cfg : compiled_shapes()) {
+        c->walk_cfg = cfg;
+        st = tvs_run_sectors(c, first, count, TVS_METHOD_BATCHED);   // warm-up
+        float total = 0.0f;
+        for (uint32_t r = 0; r < reps && st == TVS_OK; ++r) {
+            st = tvs_run_sectors(c, first, count, TVS_METHOD_BATCHED);
+            tvs_timing t;
+            if (st == TVS_OK) st = tvs_get_timing(c, &t);
+            total += t.walk_ms;
+        }
+        if (st != TVS_OK) break;
+        const float ms = total / static_cast<float>(reps);
+        if (results && k < max_results)
+            results[k] = tvs_walk_shape{cfg.block, cfg.unroll, cfg.xcd_contiguous ? 1u : 0u, ms};
+        ++k;
+        if (ms < winner_ms) {
+            winner_ms = ms;
+            winner = cfg;
+        }
+    }
+    c->walk_cfg = st == TVS_OK ? winner : keep;
+    e = hipMemcpyAsync(c->cum, saved, N * sizeof(float), hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->cum64, saved64, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(saved);
+    (void)hipFree(saved64);
+    if (st != TVS_OK) return st;
+    if (e != hipSuccess) return tfail(c, TVS_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+    if (n_results) *n_results = k;
+    if (best)
+        *best = tvs_walk_shape{winner.block, winner.unroll, winner.xcd_contiguous ? 1u : 0u,
+                               winner_ms};
+    return TVS_OK;
+}
+
+}  // extern "C"

@@ -93,6 +93,19 @@ class BtHeader(ctypes.Structure):
     ]
 
 
+class WalkShape(ctypes.Structure):
+    _fields_ = [
+        ("block", ctypes.c_uint32),
+        ("unroll", ctypes.c_uint32),
+        ("xcd_contiguous", ctypes.c_uint32),
+        ("ms", ctypes.c_float),
+    ]
+
+    def as_dict(self) -> dict:
+        return {"block": self.block, "unroll": self.unroll,
+                "xcd_contiguous": bool(self.xcd_contiguous), "ms": self.ms}
+
+
 class TvsError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"{message} (status {status})")
@@ -147,6 +160,25 @@ def load() -> ctypes.CDLL:
         "tvs_band_cache_save": (st, [vp, ctypes.c_char_p, ctypes.c_uint32]),
         "tvs_band_cache_load": (st, [vp, ctypes.c_char_p, ctypes.c_uint32]),
         "tvs_io_last_error": (ctypes.c_char_p, []),
+        "tvs_set_walk_shape": (st, [vp, P(WalkShape)]),
+        "tvs_get_walk_shape": (st, [vp, P(WalkShape)]),
+        "tvs_walk_shape_count": (ctypes.c_size_t, []),
+        "tvs_autotune": (st, [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                              P(WalkShape), ctypes.c_size_t, P(ctypes.c_size_t), P(WalkShape)]),
+        "tvs_batch_create": (st, [P(Params), ctypes.c_uint32, P(vp)]),
+        "tvs_batch_destroy": (st, [vp]),
+        "tvs_batch_tile_geometry": (st, [vp, ctypes.c_uint32, P(Geometry)]),
+        "tvs_batch_upload_i16": (st, [vp, ctypes.c_uint32, vp, ctypes.c_size_t]),
+        "tvs_batch_upload_f32": (st, [vp, ctypes.c_uint32, vp, ctypes.c_size_t]),
+        "tvs_batch_compute_bands": (st, [vp]),
+        "tvs_batch_set_walk_shape": (st, [vp, P(WalkShape)]),
+        "tvs_batch_run": (st, [vp, ctypes.c_uint32]),
+        "tvs_batch_sync": (st, [vp]),
+        "tvs_batch_reset_surfaces": (st, [vp]),
+        "tvs_batch_read_surfaces": (st, [vp, ctypes.c_uint32, vp, ctypes.c_size_t]),
+        "tvs_batch_read_surfaces_f64": (st, [vp, ctypes.c_uint32, vp, ctypes.c_size_t]),
+        "tvs_batch_get_timing": (st, [vp, P(Timing)]),
+        "tvs_batch_last_error": (ctypes.c_char_p, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -313,6 +345,25 @@ class Engine:
         self._c(load().tvs_ring_overflow_count(self._h, ctypes.byref(v)))
         return int(v.value)
 
+    def set_walk_shape(self, block: int, unroll: int, xcd_contiguous: bool = True) -> None:
+        self._c(load().tvs_set_walk_shape(self._h, ctypes.byref(
+            WalkShape(block, unroll, int(xcd_contiguous), 0.0))))
+
+    def walk_shape(self) -> dict:
+        w = WalkShape()
+        self._c(load().tvs_get_walk_shape(self._h, ctypes.byref(w)))
+        return w.as_dict()
+
+    def autotune(self, first: int = 0, count: int = 180, reps: int = 2):
+        """Time every compiled launch shape on this DEM, keep the fastest."""
+        n = int(load().tvs_walk_shape_count())
+        res = (WalkShape * n)()
+        got = ctypes.c_size_t(0)
+        best = WalkShape()
+        self._c(load().tvs_autotune(self._h, first, count, reps, res, n, ctypes.byref(got),
+                                    ctypes.byref(best)))
+        return best.as_dict(), [res[i].as_dict() for i in range(got.value)]
+
     def cache_save(self, state_dir: str, sector: int) -> None:
         self._c(load().tvs_band_cache_save(self._h, state_dir.encode(), sector))
 
@@ -325,4 +376,88 @@ class Engine:
     def timing(self) -> RunTiming:
         t = Timing()
         self._c(load().tvs_get_timing(self._h, ctypes.byref(t)))
+        return RunTiming(t.walk_ms, t.reduce_ms, t.bands_ms, t.walk_launches, t.point_steps)
+
+
+class Batch:
+    """Many DEM tiles whose 180 sectors run in one launch (tvs_batch)."""
+
+    def __init__(self, tiles):
+        """tiles: iterable of (dem_width, scale, max_line_of_sight,
+        rings_per_km, observer_height)."""
+        tiles = list(tiles)
+        arr = (Params * len(tiles))(*[Params(w, s, los, rpk, obs, 0)
+                                      for (w, s, los, rpk, obs) in tiles])
+        h = ctypes.c_void_p()
+        st = load().tvs_batch_create(arr, len(tiles), ctypes.byref(h))
+        if st != OK:
+            raise TvsError(st, (load().tvs_batch_last_error(None) or b"").decode())
+        self._h = h
+        self.n_tiles = len(tiles)
+        self.geometry = []
+        for i in range(self.n_tiles):
+            g = Geometry()
+            self._c(load().tvs_batch_tile_geometry(self._h, i, ctypes.byref(g)))
+            self.geometry.append(g)
+
+    def _c(self, status: int) -> None:
+        if status != OK:
+            raise TvsError(status, (load().tvs_batch_last_error(self._h) or b"").decode())
+
+    def close(self) -> None:
+        if self._h:
+            load().tvs_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, tile: int, elevations: np.ndarray) -> None:
+        if elevations.dtype == np.int16:
+            a = np.ascontiguousarray(elevations.reshape(-1))
+            self._c(load().tvs_batch_upload_i16(self._h, tile, _ptr(a), a.size))
+        else:
+            a = np.ascontiguousarray(elevations.reshape(-1), np.float32)
+            self._c(load().tvs_batch_upload_f32(self._h, tile, _ptr(a), a.size))
+
+    def compute_bands(self) -> None:
+        self._c(load().tvs_batch_compute_bands(self._h))
+
+    def set_walk_shape(self, block: int, unroll: int, xcd_contiguous: bool = False) -> None:
+        self._c(load().tvs_batch_set_walk_shape(self._h, ctypes.byref(
+            WalkShape(block, unroll, int(xcd_contiguous), 0.0))))
+
+    def run(self, use_graph: bool = True) -> None:
+        self._c(load().tvs_batch_run(self._h, int(use_graph)))
+
+    def sync(self) -> None:
+        self._c(load().tvs_batch_sync(self._h))
+
+    def reset_surfaces(self) -> None:
+        self._c(load().tvs_batch_reset_surfaces(self._h))
+
+    def surfaces(self, tile: int) -> np.ndarray:
+        n = self.geometry[tile].computable_points_count
+        out = np.empty(max(n, 1), np.float32)
+        self._c(load().tvs_batch_read_surfaces(self._h, tile, _ptr(out), n))
+        return out[:n]
+
+    def surfaces_f64(self, tile: int) -> np.ndarray:
+        n = self.geometry[tile].computable_points_count
+        out = np.empty(max(n, 1), np.float64)
+        self._c(load().tvs_batch_read_surfaces_f64(self._h, tile, _ptr(out), n))
+        return out[:n]
+
+    def timing(self) -> RunTiming:
+        t = Timing()
+        self._c(load().tvs_batch_get_timing(self._h, ctypes.byref(t)))
         return RunTiming(t.walk_ms, t.reduce_ms, t.bands_ms, t.walk_launches, t.point_steps)
