@@ -1,0 +1,99 @@
+"""The `total-viewsheds` CLI end to end (config.rs flags, main.rs flow) and
+the band-data cache (cache.rs layout), on the 201 x 201 workload."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import tvs_amd
+from tvs_amd import bt, demgen, _lib
+
+from test_io import read_png_grey
+
+pytestmark = pytest.mark.gpu
+CLI = os.path.join(os.path.dirname(tvs_amd.LIB_PATH), "total-viewsheds")
+
+
+def png_of(surfaces, T):
+    data = surfaces.reshape(T, T).astype(np.float32)
+    mn, mx = data.min(), data.max()
+    rng = np.float32(1.0) if mx - mn == 0 else np.float32(mx - mn)
+    return np.clip((data - mn) / rng * np.float32(255.0), 0, 255).astype(np.uint8)
+
+
+@pytest.fixture(scope="module")
+def small(oracle):
+    wl = demgen.small_201(seed=3)
+    dem = oracle.dem_new(wl.width, wl.spacing, wl.max_line_of_sight)
+    cum, _, _ = oracle.compute_run(dem, wl.elevations.astype(np.float32), wl.observer_height,
+                                   wl.rings_per_km, rings=False, n_threads=8)
+    return wl, dem, cum
+
+
+@pytest.mark.parametrize("method", ["hip", "hip-plain"])
+def test_cli_heatmap(tmp_path, small, method):
+    wl, dem, cum = small
+    path = tmp_path / "dem.bt"
+    bt.write(str(path), wl.elevations.reshape(wl.width, wl.width))
+    state = tmp_path / "state"
+    out = tmp_path / "out"
+    args = [CLI, "--input", str(path), "--scale", "100", "--max-line-of-sight", "6700",
+            "--compute", method, "--output-dir", str(out), "--state-dir", str(state)]
+    print(" ".join(args), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    np.testing.assert_array_equal(read_png_grey(str(out / "heatmap.png")),
+                                  png_of(cum, dem.tvs_width))
+    cache = state / "total-viewsheds" / "201"
+    assert len(os.listdir(cache / "band_deltas")) == 180
+    assert os.path.getsize(cache / "distances" / "17.bin") == 4 * dem.n_deltas
+    # Second run reads the cache and draws the same picture.
+    r2 = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r2.returncode == 0, r2.stderr
+    assert "Computing band data" not in r2.stderr
+    np.testing.assert_array_equal(read_png_grey(str(out / "heatmap.png")),
+                                  png_of(cum, dem.tvs_width))
+
+
+def test_cli_default_line_of_sight_and_errors(tmp_path, small):
+    wl, dem, _ = small
+    path = tmp_path / "dem.bt"
+    bt.write(str(path), wl.elevations.reshape(wl.width, wl.width))
+    # main.rs:36-38: default LoS = (w div 3) * scale = 6700 m at 100 m.
+    r = subprocess.run([CLI, "--input", str(path), "--scale", "100", "--no-cache",
+                        "--output-dir", str(tmp_path)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "max line of sight 6700 m" in r.stderr
+    r = subprocess.run([CLI, "--input", str(path), "--scale", "100",
+                        "--max-line-of-sight", "20000", "--no-cache"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "longer than the maximum distance" in r.stderr
+    r = subprocess.run([CLI, "--input", str(path), "--compute", "vulkan"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2
+
+
+def test_band_cache_round_trip(tmp_path, small):
+    wl, dem, cum = small
+    with tvs_amd.Engine(wl.width, wl.spacing, wl.max_line_of_sight, wl.rings_per_km,
+                        wl.observer_height) as e:
+        e.upload(wl.elevations)
+        e.compute_bands(0, 180)
+        for s in range(180):
+            e.cache_save(str(tmp_path), s)
+    with tvs_amd.Engine(wl.width, wl.spacing, wl.max_line_of_sight, wl.rings_per_km,
+                        wl.observer_height) as e:
+        e.upload(wl.elevations)
+        for s in range(180):
+            e.cache_load(str(tmp_path), s)
+        e.run(0, 180, "batched")
+        np.testing.assert_array_equal(e.surfaces(), cum)
+    # A file of another band length (another line of sight) is refused.
+    f = tmp_path / "total-viewsheds" / "201" / "distances" / "5.bin"
+    f.write_bytes(f.read_bytes()[:-4])
+    with tvs_amd.Engine(wl.width, wl.spacing, wl.max_line_of_sight) as e:
+        with pytest.raises(tvs_amd.TvsError) as err:
+            e.cache_load(str(tmp_path), 5)
+        assert err.value.status == _lib.ERR_SIZE
