@@ -83,6 +83,67 @@ def cpu_baseline(wl, sectors: int, threads: int):
                        f"CPU loop, {threads} thread(s), {dt:.1f} s on {cpu}")}
 
 
+def copy_calibration(mib: int) -> int:
+    """A plain device-to-device copy of known size (the runtime's copy
+    kernel), to scale the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0   # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
+    return 0
+
+
+def _counter_rows(path: str):
+    import csv
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            yield row
+
+
+def summarize_pmc(fetch_csv: str, write_csv: str, copy_fetch_csv: str,
+                  copy_write_csv: str, copy_bytes: int, out_path: str) -> int:
+    """HBM bytes per launch of the batched walk from two counter runs
+    (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by what the same
+    counter reads for a copy of known size."""
+    def per_kernel(path, counter):
+        vals = {}
+        for r in _counter_rows(path):
+            if r.get("Counter_Name") != counter:
+                continue
+            name = r.get("Kernel_Name", "")
+            vals.setdefault(name, []).append(float(r["Counter_Value"]))
+        return vals
+
+    def pick(vals, needle):
+        hits = [v for k, v in vals.items() if needle in k]
+        if not hits:
+            raise SystemExit(f"no {needle} dispatch in counters")
+        return float(np.mean(hits[0]))
+
+    kib = copy_bytes / 1024.0
+    fetch_scale = pick(per_kernel(copy_fetch_csv, "FETCH_SIZE"), "copyBuffer") / kib
+    write_scale = pick(per_kernel(copy_write_csv, "WRITE_SIZE"), "copyBuffer") / kib
+    fetch = pick(per_kernel(fetch_csv, "FETCH_SIZE"), "walk_tiles_kernel")
+    write = pick(per_kernel(write_csv, "WRITE_SIZE"), "walk_tiles_kernel")
+    res = {"kernel": "walk_tiles_kernel (batched walk, main-1201)",
+           "fetch_kib_raw": fetch, "write_kib_raw": write,
+           "copy_fetch_scale": fetch_scale, "copy_write_scale": write_scale,
+           "hbm_bytes_per_launch": (fetch / fetch_scale + write / write_scale) * 1024.0}
+    with open(out_path, "w") as f:
+        json.dump(res, f, indent=1)
+    print(json.dumps(res))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__,
                                  formatter_class=argparse.RawDescriptionHelpFormatter)
@@ -93,9 +154,22 @@ def main() -> int:
                     choices=["main-1201", "small-201", "large-3601"])
     ap.add_argument("--no-plain", action="store_true", help="skip the plain method")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default walk shape")
+    ap.add_argument("--no-f32", action="store_true", help="skip the f32-heights batched run")
     ap.add_argument("--cpu-sectors", type=int, default=18)
     ap.add_argument("--cpu-threads", type=int, default=1)
+    ap.add_argument("--copy-calibration", type=int, metavar="MIB",
+                    help="only run a device copy of MIB MiB (counter calibration)")
+    ap.add_argument("--summarize-pmc", nargs=6,
+                    metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV",
+                             "COPY_BYTES", "OUT_JSON"),
+                    help="turn counter CSVs into HBM bytes per walk launch")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
+    if args.summarize_pmc:
+        a = args.summarize_pmc
+        return summarize_pmc(a[0], a[1], a[2], a[3], int(a[4]), a[5])
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine runs on a single MI355X "
               "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
@@ -113,8 +187,16 @@ def main() -> int:
     e.compute_bands(0, 180)
     bands_wall_ms = (time.perf_counter() - t0) * 1000.0
     bands_ms = e.timing().bands_ms
-    log(f"band data for 180 sectors: {bands_ms:.1f} ms device, {bands_wall_ms:.1f} ms wall")
+    log(f"band data for 180 sectors: {bands_ms:.1f} ms device, {bands_wall_ms:.1f} ms wall, "
+        f"{e.bands_sorted_count()} sectors through the sorts")
     point_steps = 180 * 2 * g.computable_points_count * g.band_deltas_size
+    real_point_steps = 180 * 2 * g.computable_points_count * g.real_steps
+
+    shape = e.walk_shape()
+    if not args.no_tune:
+        shape, table = e.autotune(0, 180, reps=1)
+        log(f"autotuned walk shape: block {shape['block']}, unroll {shape['unroll']}, "
+            f"xcd-contiguous {shape['xcd_contiguous']} ({shape['ms']:.2f} ms)")
 
     plain = None
     if not args.no_plain:
@@ -130,6 +212,20 @@ def main() -> int:
     value = point_steps / (b_ms / 1000.0)
     log(f"batched: {b_ms:.2f} ms/step (walk kernel {b_kernel_ms:.2f} ms, reduce "
         f"{tm.reduce_ms:.2f} ms), {value:.3e} point-steps/s")
+
+    batched_f32 = None
+    if not args.no_f32:
+        log("timing batched with f32 heights")
+        with tvs_amd.Engine(wl.width, wl.spacing, wl.max_line_of_sight, wl.rings_per_km,
+                            wl.observer_height, 0) as ef:
+            ef.upload(wl.elevations.astype(np.float32))
+            ef.compute_bands(0, 180)
+            ef.set_walk_shape(shape["block"], shape["unroll"], shape["xcd_contiguous"])
+            f_ms, f_kernel_ms, _ = time_method(ef, "batched", args.steps, args.warmup)
+        batched_f32 = {"value": point_steps / (f_ms / 1000.0), "ms_per_step": f_ms,
+                       "kernel_ms": f_kernel_ms,
+                       "achieved_GBps": point_steps * 4 / (f_kernel_ms / 1000.0) / 1e9}
+        log(f"batched f32: {f_ms:.2f} ms/step, {batched_f32['value']:.3e} point-steps/s")
 
     # Correctness of the timed configuration: one fresh pass of each method.
     e.reset_surfaces()
@@ -147,7 +243,7 @@ def main() -> int:
     bytes_per_launch = point_steps * 2          # one int16 height per point-step
     achieved = bytes_per_launch / (b_kernel_ms / 1000.0) / 1e9
     traffic = None
-    prof = os.path.join(ROOT, "profiles", "walk_batched_main-1201_traffic.json")
+    prof = os.path.join(ROOT, "profiles", "walk_tiles_main-1201_traffic.json")
     if os.path.exists(prof):
         with open(prof) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -178,6 +274,7 @@ def main() -> int:
             "point_steps_per_step": point_steps,
             "steps_per_band": g.band_deltas_size,
             "real_steps_per_band": g.real_steps,
+            "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous")},
         },
         "roofline": {
             "bound": "hbm",
@@ -186,11 +283,13 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": traffic,
-            "kernel": "walk_batched_kernel<int16_t, fast divide>",
+            "kernel": "walk_tiles_kernel<int16_t> (batched walk)",
             "kernel_ms": b_kernel_ms,
             "bytes_per_launch": bytes_per_launch,
         },
+        "real_point_steps_per_s": real_point_steps / (b_ms / 1000.0),
         "plain": plain,
+        "batched_f32": batched_f32,
         "tvs_rel_err_batched_vs_plain": rel_err,
         "tvs_bit_exact_batched_vs_plain": bit_exact,
         "tvs_rel_err_f64_sum_vs_plain": rel_err64,

@@ -136,11 +136,28 @@ tvs_status tvs_upload_dem_i16(tvs_ctx *ctx, const int16_t *elevations,
 
 /* `DEM::calculate_axes` + `DEM::compile_band_data` (dem.rs:135-139,
  * axes.rs:103-130, band_of_sight.rs:61-173) for sectors
- * [first, first+count), on the device: f64 rotated-coordinate keys, stable
- * LSD radix sorts, inverse sight map, band compile.  The band data stays
- * resident for the walk.  Synchronous. */
+ * [first, first+count), on the device, from f64 rotated-coordinate keys
+ * (see tvs_band_method for the two ways of ordering them).  The band data
+ * stays resident for the walk.  Synchronous. */
 tvs_status tvs_compute_bands(tvs_ctx *ctx, uint32_t first_sector,
                              uint32_t sector_count);
+
+/* How tvs_compute_bands finds each sector's window and its sight order.
+ * SELECT (default): per-row binary searches over the monotone rows of the
+ * rotated key pick the window exactly, and the samples are ordered by
+ * (sight key, id) -- the same data the two stable sorts give, without
+ * them.  SORT: the two full stable radix sorts of axes.rs:103-130 and the
+ * inverse sight map, as tvs_compute_axes runs them.  SELECT falls back to
+ * SORT for a sector whose window plus key ties exceeds its buffer. */
+typedef enum tvs_band_method {
+    TVS_BANDS_SELECT = 0,
+    TVS_BANDS_SORT = 1
+} tvs_band_method;
+
+tvs_status tvs_set_band_method(tvs_ctx *ctx, tvs_band_method method);
+
+/* Sectors of the last tvs_compute_bands call that went through the sorts. */
+tvs_status tvs_bands_sorted_count(const tvs_ctx *ctx, uint32_t *out);
 
 /* Band data supplied from outside (a cache file, another producer): the
  * inputs of `GPU::run` (gpu.rs:387-394), `n` = band_deltas_size. */

@@ -44,11 +44,15 @@ def make_engine(wl, capture=0, rings_per_km=None):
                           wl.observer_height, capture)
 
 
-def test_small_band_data_bit_exact(small):
+@pytest.mark.parametrize("band_method", ["select", "sort"])
+def test_small_band_data_bit_exact(small, band_method):
     wl, dem, deltas, dists, _ = small
-    note("small-201: device sort + band compile, 180 sectors")
+    note(f"small-201: device band data ({band_method}), 180 sectors")
     with make_engine(wl) as e:
+        e.set_band_method(band_method)
         e.compute_bands(0, 180)
+        if band_method == "select":
+            assert e.bands_sorted_count() == 0
         for s in range(180):
             d, x = e.get_band(s)
             np.testing.assert_array_equal(d, deltas[s], err_msg=f"deltas sector {s}")
@@ -136,6 +140,7 @@ def main(oracle):
     e = make_engine(wl)
     e.upload(wl.elevations)
     e.compute_bands(0, 180)
+    note(f"main-1201 bands: {e.timing().bands_ms:.1f} ms, {e.bands_sorted_count()} sorted")
     dem = oracle.dem_new(wl.width, wl.spacing, wl.max_line_of_sight)
     yield wl, dem, e
     e.close()
@@ -146,6 +151,14 @@ SAMPLE = [0, 1, 2, 44, 45, 46, 89, 90, 91, 134, 135, 136, 178, 179]
 
 def test_main_band_data_bit_exact(main, oracle):
     wl, dem, e = main
+    with make_engine(wl) as es:
+        es.set_band_method("sort")
+        es.compute_bands(0, 180)
+        note(f"main-1201 bands by sorts: {es.timing().bands_ms:.1f} ms")
+        for s in range(180):       # the two device paths agree everywhere
+            a, b = e.get_band(s), es.get_band(s)
+            np.testing.assert_array_equal(a[0], b[0], err_msg=f"sector {s}")
+            np.testing.assert_array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
     for s in SAMPLE:
         note(f"main-1201 band sector {s}")
         want_d, want_x = oracle.band_data(dem, s)

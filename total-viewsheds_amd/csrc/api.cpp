@@ -5,6 +5,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "engine.hpp"
 
@@ -122,7 +123,7 @@ tvs_status ensure_sort_scratch(tvs_ctx *c) {
     TVS_HIP(c, dev_alloc(&s.id_b, n), "sort ids");
     TVS_HIP(c, dev_alloc(&s.sight_map, n), "sight map");
     TVS_HIP(c, dev_alloc(&s.dist_f32, n), "axes distances");
-    TVS_HIP(c, dev_alloc(&s.hist, static_cast<size_t>(256) * tiles), "sort histogram");
+    TVS_HIP(c, dev_alloc(&s.hist, static_cast<size_t>(256) * (tiles + 1)), "sort histogram");
     s.n = n;
     s.tiles = tiles;
     return TVS_OK;
@@ -294,6 +295,17 @@ tvs_status tvs_create(const tvs_params *p, tvs_ctx **out) {
     if (e == hipSuccess) e = dev_alloc(&c->cum64, N);
     if (e == hipSuccess) e = dev_alloc(&c->overflow, 1);
     if (e == hipSuccess) e = dev_alloc(&c->d_tile, 1);
+    if (e == hipSuccess) e = dev_alloc(&c->d_axes, 4u * tvs::kSectorSteps);
+    if (e == hipSuccess) e = dev_alloc(&c->d_select_status, tvs::kSectorSteps);
+    if (e == hipSuccess) {
+        std::vector<double> axes(4u * tvs::kSectorSteps);
+        for (uint32_t s = 0; s < tvs::kSectorSteps; ++s) {
+            tvs::sector_sin_cos(s, false, &axes[4 * s], &axes[4 * s + 1]);
+            tvs::sector_sin_cos(s, true, &axes[4 * s + 2], &axes[4 * s + 3]);
+        }
+        e = hipMemcpy(c->d_axes, axes.data(), axes.size() * sizeof(double),
+                      hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess) e = dev_alloc(&c->d_begin, 4);
     c->tab.stride = std::max<uint32_t>(64u, (g.band_deltas_size + 63u) / 64u * 64u);
     const size_t tab_n = static_cast<size_t>(tvs::kBandSlots) * c->tab.stride;
@@ -339,6 +351,8 @@ tvs_status tvs_destroy(tvs_ctx *c) {
     dev_free(c->ring);
     dev_free(c->overflow);
     dev_free(c->d_tile);
+    dev_free(c->d_axes);
+    dev_free(c->d_select_status);
     dev_free(c->d_begin);
     free_sort_scratch(c->sort);
     for (auto &ev : c->ev)
@@ -409,12 +423,29 @@ tvs_status tvs_compute_bands(tvs_ctx *c, uint32_t first, uint32_t count) {
     if (count == 0) return TVS_OK;
     tvs_status st = check_band_geometry(c);
     if (st != TVS_OK) return st;
-    if ((st = ensure_sort_scratch(c)) != TVS_OK) return st;
     TVS_HIP(c, hipEventRecord(c->ev[4], c->stream), "event");
-    for (uint32_t s = first; s < first + count; ++s) {
-        TVS_HIP(c, tvs::launch_axes(c, s, false), "axes sort");
-        TVS_HIP(c, tvs::launch_band_compile(c, s), "band compile");
+    std::vector<uint32_t> sorted_path;
+    if (c->band_path == TVS_BANDS_SELECT) {
+        TVS_HIP(c, tvs::launch_band_select(c, first, count, c->d_axes, c->d_select_status),
+                "band selection");
+        std::vector<uint32_t> status(count, 0);
+        TVS_HIP(c, hipMemcpyAsync(status.data(), c->d_select_status + first,
+                                  count * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream),
+                "band selection status");
+        TVS_HIP(c, hipStreamSynchronize(c->stream), "band selection");
+        for (uint32_t i = 0; i < count; ++i)
+            if (status[i]) sorted_path.push_back(first + i);
+    } else {
+        for (uint32_t s = first; s < first + count; ++s) sorted_path.push_back(s);
     }
+    if (!sorted_path.empty()) {
+        if ((st = ensure_sort_scratch(c)) != TVS_OK) return st;
+        for (uint32_t s : sorted_path) {
+            TVS_HIP(c, tvs::launch_axes(c, s, false), "axes sort");
+            TVS_HIP(c, tvs::launch_band_compile(c, s), "band compile");
+        }
+    }
+    c->bands_sorted_sectors = static_cast<uint32_t>(sorted_path.size());
     TVS_HIP(c, tvs::launch_step_tables(c, first, count), "step tables");
     TVS_HIP(c, hipEventRecord(c->ev[5], c->stream), "event");
     TVS_HIP(c, hipEventSynchronize(c->ev[5]), "band computation");
@@ -422,6 +453,20 @@ tvs_status tvs_compute_bands(tvs_ctx *c, uint32_t first, uint32_t count) {
     TVS_HIP(c, hipEventElapsedTime(&ms, c->ev[4], c->ev[5]), "event time");
     c->timing.bands_ms = ms;
     return read_meta(c, first, count);
+}
+
+tvs_status tvs_set_band_method(tvs_ctx *c, tvs_band_method m) {
+    if (!c) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null context");
+    if (m != TVS_BANDS_SELECT && m != TVS_BANDS_SORT)
+        return fail(c, TVS_ERR_INVALID_ARGUMENT, "unknown band method");
+    c->band_path = m;
+    return TVS_OK;
+}
+
+tvs_status tvs_bands_sorted_count(const tvs_ctx *c, uint32_t *out) {
+    if (!c || !out) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null argument");
+    *out = c->bands_sorted_sectors;
+    return TVS_OK;
 }
 
 tvs_status tvs_compute_axes(tvs_ctx *c, uint32_t sector, float *distances,

@@ -51,8 +51,9 @@ __global__ void keys_kernel(uint32_t w, double s, double c,
     }
 }
 
-// LSD radix sort, 8-bit digits.  Pass = histogram per tile -> one exclusive
-// scan over the digit-major [256][tiles] table -> stable scatter.
+// LSD radix sort, 8-bit digits.  Pass = histogram per tile -> per-digit
+// exclusive scans over the digit-major [256][tiles] table (+ 256 totals) ->
+// stable scatter.
 __global__ __launch_bounds__(kSortThreads) void sort_hist_kernel(
     const uint32_t *__restrict__ key, uint32_t n, uint32_t shift,
     uint32_t *__restrict__ hist, uint32_t tiles) {
@@ -70,14 +71,17 @@ __global__ __launch_bounds__(kSortThreads) void sort_hist_kernel(
     hist[d * tiles + blockIdx.x] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
 }
 
-__global__ __launch_bounds__(kScanThreads) void sort_scan_kernel(
-    uint32_t *__restrict__ data, uint32_t m) {
+// One block per digit: exclusive scan of that digit's per-tile counts (in
+// place) and the digit's total.
+__global__ __launch_bounds__(kScanThreads) void sort_rowscan_kernel(
+    uint32_t *__restrict__ hist, uint32_t tiles, uint32_t *__restrict__ totals) {
     __shared__ uint32_t part[kScanThreads];
-    const uint32_t chunk = (m + kScanThreads - 1) / kScanThreads;
+    uint32_t *row = hist + static_cast<size_t>(blockIdx.x) * tiles;
+    const uint32_t chunk = (tiles + kScanThreads - 1) / kScanThreads;
     const uint32_t begin = threadIdx.x * chunk;
-    const uint32_t end = begin + chunk < m ? begin + chunk : m;
+    const uint32_t end = begin + chunk < tiles ? begin + chunk : tiles;
     uint32_t local = 0;
-    for (uint32_t i = begin; i < end; ++i) local += data[i];
+    for (uint32_t i = begin; i < end; ++i) local += row[i];
     part[threadIdx.x] = local;
     __syncthreads();
     for (uint32_t o = 1; o < kScanThreads; o <<= 1) {
@@ -88,23 +92,36 @@ __global__ __launch_bounds__(kScanThreads) void sort_scan_kernel(
     }
     uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
     for (uint32_t i = begin; i < end; ++i) {
-        const uint32_t v = data[i];
-        data[i] = run;
+        const uint32_t v = row[i];
+        row[i] = run;
         run += v;
     }
+    if (threadIdx.x == kScanThreads - 1) totals[blockIdx.x] = part[kScanThreads - 1];
 }
 
 __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(
     const uint32_t *__restrict__ key_in, const uint32_t *__restrict__ id_in,
     uint32_t *__restrict__ key_out, uint32_t *__restrict__ id_out, uint32_t n,
-    uint32_t shift, const uint32_t *__restrict__ offsets, uint32_t tiles) {
+    uint32_t shift, const uint32_t *__restrict__ offsets, uint32_t tiles,
+    const uint32_t *__restrict__ totals) {
     __shared__ uint32_t run[256];
     __shared__ uint32_t wave_count[4][256];
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid >> 6;
     const uint32_t lane = tid & 63u;
     const uint64_t lanes_below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
-    run[tid] = offsets[tid * tiles + blockIdx.x];
+    // Start of digit `tid` in the output: exclusive scan of the digit totals.
+    run[tid] = totals[tid];
+    __syncthreads();
+    for (uint32_t o = 1; o < 256; o <<= 1) {
+        const uint32_t v = tid >= o ? run[tid - o] : 0u;
+        __syncthreads();
+        run[tid] += v;
+        __syncthreads();
+    }
+    const uint32_t digit_base = run[tid] - totals[tid];
+    __syncthreads();
+    run[tid] = digit_base + offsets[tid * tiles + blockIdx.x];
     for (uint32_t wv = 0; wv < 4; ++wv) wave_count[wv][tid] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * kSortTile;
@@ -281,6 +298,217 @@ __global__ __launch_bounds__(kScanThreads) void step_table_kernel(
     }
 }
 
+
+// ---- band data without the full sorts --------------------------------------
+//
+// compile_band_data (band_of_sight.rs:95-169) reads two things from the axes:
+// the `band_samples` points at ranks [P - bs/2, P - bs/2 + bs) of the sector
+// order, and the relative sight order of those samples.  Both orders sort by
+// (f32 key in IEEE total order, id), so:
+//  * within a DEM row the sector key is monotone in x (x*sin - y*cos with y
+//    fixed; every rounding step is monotone), which makes
+//    count(key <= K) a sum of per-row binary searches, and the keys at the
+//    window's first and last rank follow by bisection over the 32-bit key;
+//  * the points with key between those two keys are few (the window plus
+//    ties), sorted by (key, id) in LDS; their global ranks start at
+//    count(key < K_lo), which cuts the window out exactly;
+//  * the samples' sight order is their (sight key, id) order.
+// One block per sector; all sectors of a call in one launch.
+
+constexpr uint32_t kSelectThreads = 1024;
+constexpr uint32_t kSelectCapacity = 8192;   // candidates held in LDS
+
+struct SectorAxes {
+    double s_sight, c_sight;     // sight axis (a = sector + 0.001)
+    double s_sector, c_sector;   // sector axis (a + 90)
+};
+
+__device__ __forceinline__ uint32_t sector_key(uint32_t r, uint32_t x, double s, double c) {
+    const double y = static_cast<double>(-static_cast<int64_t>(r));
+    const double left = static_cast<double>(x) * s;
+    const double right = y * c;
+    return total_order_key(static_cast<float>(left - right));
+}
+
+// Number of x in [0, w) of row r whose key is <= K (rows are non-decreasing
+// in x when s >= 0, non-increasing otherwise).
+__device__ __forceinline__ uint32_t row_count_le(uint32_t r, uint32_t w, double s, double c,
+                                                 bool increasing, uint32_t K) {
+    uint32_t lo = 0, hi = w;
+    if (increasing) {
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (sector_key(r, mid, s, c) <= K) lo = mid + 1; else hi = mid;
+        }
+        return lo;
+    }
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (sector_key(r, mid, s, c) <= K) hi = mid; else lo = mid + 1;
+    }
+    return w - lo;
+}
+
+__device__ uint32_t block_sum(uint32_t v, uint32_t *scratch) {
+    for (uint32_t o = 32; o > 0; o >>= 1) v += __shfl_xor(static_cast<int>(v), o);
+    __syncthreads();
+    if ((threadIdx.x & 63u) == 0) scratch[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t t = 0;
+    for (uint32_t i = 0; i < kSelectThreads / 64; ++i) t += scratch[i];
+    return t;
+}
+
+__device__ void bitonic_sort(uint64_t *a, uint32_t n) {
+    for (uint32_t k = 2; k <= n; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+                const uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const uint64_t x = a[i], y = a[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((x > y) == up) {
+                        a[i] = y;
+                        a[ixj] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ __launch_bounds__(kSelectThreads) void band_select_kernel(
+    const SectorAxes *__restrict__ axes, uint32_t first_sector, uint32_t w,
+    uint32_t band_size, uint32_t band_samples, int32_t *__restrict__ deltas_base,
+    float *__restrict__ dists_base, uint32_t stride, uint32_t *__restrict__ status) {
+    __shared__ uint64_t cand[kSelectCapacity];
+    __shared__ uint32_t red[kSelectThreads / 64];
+    __shared__ uint32_t row_base[kSelectThreads + 1];
+    const uint32_t sector = first_sector + blockIdx.x;
+    const SectorAxes ax = axes[sector];
+    const double s = ax.s_sector, c = ax.c_sector;
+    const bool inc = s >= 0.0;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t n = w * w;
+    const uint32_t pov_id = n / 2u;
+    const uint32_t edge = pov_id - band_samples / 2u;
+    const uint32_t last = edge + band_samples - 1u;
+
+    // Keys at ranks `edge` and `last`: smallest K with count(key <= K) > rank.
+    uint32_t lo1 = 0, hi1 = 0xffffffffu, lo2 = 0, hi2 = 0xffffffffu;
+    while (lo1 < hi1 || lo2 < hi2) {
+        const uint32_t m1 = lo1 + ((hi1 - lo1) >> 1);
+        const uint32_t m2 = lo2 + ((hi2 - lo2) >> 1);
+        uint32_t c1 = 0, c2 = 0;
+        for (uint32_t r = tid; r < w; r += kSelectThreads) {
+            if (lo1 < hi1) c1 += row_count_le(r, w, s, c, inc, m1);
+            if (lo2 < hi2) c2 += row_count_le(r, w, s, c, inc, m2);
+        }
+        c1 = block_sum(c1, red);
+        c2 = block_sum(c2, red);
+        if (lo1 < hi1) { if (c1 > edge) hi1 = m1; else lo1 = m1 + 1; }
+        if (lo2 < hi2) { if (c2 > last) hi2 = m2; else lo2 = m2 + 1; }
+    }
+    const uint32_t k_lo = lo1, k_hi = lo2;
+    // Ranks start at count(key < k_lo).
+    uint32_t below = 0;
+    if (k_lo > 0)
+        for (uint32_t r = tid; r < w; r += kSelectThreads)
+            below += row_count_le(r, w, s, c, inc, k_lo - 1u);
+    below = block_sum(below, red);
+
+    // Candidates: per row the x range with k_lo <= key <= k_hi.  Rows are
+    // split in contiguous chunks per thread so ids come out row-major.
+    const uint32_t rows_per = (w + kSelectThreads - 1) / kSelectThreads;
+    const uint32_t r0 = tid * rows_per < w ? tid * rows_per : w;
+    const uint32_t r1 = r0 + rows_per < w ? r0 + rows_per : w;
+    uint32_t mine = 0;
+    for (uint32_t r = r0; r < r1; ++r) {
+        const uint32_t le_hi = row_count_le(r, w, s, c, inc, k_hi);
+        const uint32_t lt_lo = k_lo ? row_count_le(r, w, s, c, inc, k_lo - 1u) : 0u;
+        mine += le_hi - lt_lo;
+    }
+    row_base[tid + 1] = mine;
+    if (tid == 0) row_base[0] = 0;
+    __syncthreads();
+    if (tid == 0)
+        for (uint32_t i = 1; i <= kSelectThreads; ++i) row_base[i] += row_base[i - 1];
+    __syncthreads();
+    const uint32_t total = row_base[kSelectThreads];
+    uint32_t nb = 1;
+    while (nb < total) nb <<= 1;
+    if (nb > kSelectCapacity) {
+        if (tid == 0) status[sector] = 1u;   // host takes the full-sort path
+        return;
+    }
+    uint32_t pos = row_base[tid];
+    for (uint32_t r = r0; r < r1; ++r) {
+        const uint32_t le_hi = row_count_le(r, w, s, c, inc, k_hi);
+        const uint32_t lt_lo = k_lo ? row_count_le(r, w, s, c, inc, k_lo - 1u) : 0u;
+        // Increasing rows: x in [lt_lo, le_hi); decreasing: [w-le_hi, w-lt_lo).
+        const uint32_t xa = inc ? lt_lo : w - le_hi;
+        const uint32_t xb = inc ? le_hi : w - lt_lo;
+        for (uint32_t x = xa; x < xb; ++x)
+            cand[pos++] = (static_cast<uint64_t>(sector_key(r, x, s, c)) << 32) | (r * w + x);
+    }
+    for (uint32_t i = total + tid; i < nb; i += kSelectThreads) cand[i] = ~0ull;
+    __syncthreads();
+    bitonic_sort(cand, nb);
+
+    // The window, re-keyed by sight order: (sight key, id).
+    const uint32_t off = edge - below;
+    uint32_t nb2 = 1;
+    while (nb2 < band_samples) nb2 <<= 1;
+    constexpr uint32_t kPer = kSelectCapacity / kSelectThreads;
+    uint64_t held[kPer];
+#pragma unroll
+    for (uint32_t q = 0; q < kPer; ++q) {
+        const uint32_t j = tid + q * kSelectThreads;
+        held[q] = j < band_samples ? cand[off + j] : ~0ull;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t q = 0; q < kPer; ++q) {
+        const uint32_t j = tid + q * kSelectThreads;
+        if (j < nb2) {
+            uint64_t v = ~0ull;
+            if (j < band_samples) {
+                const uint32_t id = static_cast<uint32_t>(held[q]);
+                const uint32_t r = id / w;
+                const uint32_t key = sector_key(r, id - r * w, ax.s_sight, ax.c_sight);
+                v = (static_cast<uint64_t>(key) << 32) | id;
+            }
+            cand[j] = v;
+        }
+    }
+    __syncthreads();
+    bitonic_sort(cand, nb2);
+
+    const uint32_t middle = band_samples / 2u;
+    const uint32_t end_of_band = band_size * 2u;
+    const uint32_t n_steps = band_size - 1u;
+    int32_t *deltas = deltas_base + static_cast<size_t>(sector) * stride;
+    float *dists = dists_base + static_cast<size_t>(sector) * stride;
+    const float pov_distance =
+        static_cast<float>(rotated_distance(pov_id, w, ax.s_sight, ax.c_sight));
+    for (uint32_t i = tid; i < n_steps; i += kSelectThreads) {
+        int32_t delta = 0;
+        float distance = 0.0f;
+        const uint32_t band_id = middle + i;
+        if (band_id < end_of_band) {
+            const uint32_t cur = static_cast<uint32_t>(cand[band_id]);
+            const uint32_t next = static_cast<uint32_t>(cand[band_id + 1]);
+            delta = static_cast<int32_t>(cur) - static_cast<int32_t>(next);
+            distance = fabsf(pov_distance -
+                             static_cast<float>(rotated_distance(next, w, ax.s_sight, ax.c_sight)));
+        }
+        deltas[i] = delta;
+        dists[i] = distance;
+    }
+    if (tid == 0) status[sector] = 0u;
+}
+
 uint32_t grid_for(size_t n, uint32_t block) {
     size_t g = (n + block - 1) / block;
     if (g > 65536) g = 65536;
@@ -295,11 +523,11 @@ hipError_t radix_sort(tvs_ctx *c, size_t n) {
         const uint32_t shift = 8u * pass;
         hipLaunchKernelGGL(sort_hist_kernel, dim3(tiles), dim3(kSortThreads), 0, c->stream,
                            ks, static_cast<uint32_t>(n), shift, s.hist, tiles);
-        hipLaunchKernelGGL(sort_scan_kernel, dim3(1), dim3(kScanThreads), 0, c->stream,
-                           s.hist, 256u * tiles);
+        hipLaunchKernelGGL(sort_rowscan_kernel, dim3(256), dim3(kScanThreads), 0, c->stream,
+                           s.hist, tiles, s.hist + 256u * tiles);
         hipLaunchKernelGGL(sort_scatter_kernel, dim3(tiles), dim3(kSortThreads), 0,
                            c->stream, ks, is, kd, id, static_cast<uint32_t>(n), shift,
-                           s.hist, tiles);
+                           s.hist, tiles, s.hist + 256u * tiles);
         std::swap(ks, kd);
         std::swap(is, id);
     }
@@ -354,5 +582,21 @@ hipError_t launch_step_tables(tvs_ctx *c, uint32_t first_slot, uint32_t count) {
 }
 
 uint32_t max_band_samples() { return kMaxBandSamples; }
+
+// Band data of sectors [first, first+count) by selection (no full sorts).
+// `status[s]` is set non-zero for a sector whose window plus ties did not fit
+// the LDS candidate buffer; the caller computes those with the sorts.
+hipError_t launch_band_select(tvs_ctx *c, uint32_t first, uint32_t count,
+                              const void *d_axes, uint32_t *d_status) {
+    if (count == 0) return hipSuccess;
+    const tvs_geometry &g = c->geo;
+    hipLaunchKernelGGL(band_select_kernel, dim3(count), dim3(kSelectThreads), 0, c->stream,
+                       static_cast<const SectorAxes *>(d_axes), first, g.dem_width,
+                       g.band_size, g.band_samples, c->tab.delta, c->tab.dist,
+                       c->tab.stride, d_status);
+    return hipGetLastError();
+}
+
+uint32_t band_select_capacity() { return kSelectCapacity; }
 
 }  // namespace tvs

@@ -140,6 +140,10 @@ struct tvs_ctx {
     unsigned long long *overflow = nullptr;
 
     tvs::SortScratch sort;
+    double *d_axes = nullptr;      // per sector: sin/cos of sight and sector axes
+    uint32_t *d_select_status = nullptr;
+    int band_path = TVS_BANDS_SELECT;
+    uint32_t bands_sorted_sectors = 0;   // sectors of the last call that took the sorts
     tvs_timing timing{};
     std::string err;
 };
@@ -150,6 +154,9 @@ namespace tvs {
 hipError_t launch_axes(tvs_ctx *c, uint32_t sector, bool want_distances);
 hipError_t launch_band_compile(tvs_ctx *c, uint32_t sector);
 hipError_t launch_step_tables(tvs_ctx *c, uint32_t first_slot, uint32_t count);
+hipError_t launch_band_select(tvs_ctx *c, uint32_t first, uint32_t count,
+                              const void *d_axes, uint32_t *d_status);
+uint32_t band_select_capacity();
 
 // ---- launchers (walk.hip) ----
 hipError_t launch_walk_plain(tvs_ctx *c, uint32_t slot, uint32_t *ring_out);

@@ -160,6 +160,8 @@ def load() -> ctypes.CDLL:
         "tvs_band_cache_save": (st, [vp, ctypes.c_char_p, ctypes.c_uint32]),
         "tvs_band_cache_load": (st, [vp, ctypes.c_char_p, ctypes.c_uint32]),
         "tvs_io_last_error": (ctypes.c_char_p, []),
+        "tvs_set_band_method": (st, [vp, ctypes.c_int]),
+        "tvs_bands_sorted_count": (st, [vp, P(ctypes.c_uint32)]),
         "tvs_set_walk_shape": (st, [vp, P(WalkShape)]),
         "tvs_get_walk_shape": (st, [vp, P(WalkShape)]),
         "tvs_walk_shape_count": (ctypes.c_size_t, []),
@@ -288,6 +290,16 @@ class Engine:
 
     def compute_bands(self, first: int = 0, count: int = 180) -> None:
         self._c(load().tvs_compute_bands(self._h, first, count))
+
+    def set_band_method(self, method: str) -> None:
+        """'select' (default: exact window selection, no full sorts) or
+        'sort' (the two stable radix sorts of Axes::compute)."""
+        self._c(load().tvs_set_band_method(self._h, {"select": 0, "sort": 1}[method]))
+
+    def bands_sorted_count(self) -> int:
+        v = ctypes.c_uint32(0)
+        self._c(load().tvs_bands_sorted_count(self._h, ctypes.byref(v)))
+        return int(v.value)
 
     def set_band(self, sector: int, deltas: np.ndarray, dists: np.ndarray) -> None:
         d = np.ascontiguousarray(deltas, np.int32)
