@@ -196,7 +196,11 @@ def main() -> int:
     if not args.no_tune:
         shape, table = e.autotune(0, 180, reps=1)
         log(f"autotuned walk shape: block {shape['block']}, unroll {shape['unroll']}, "
-            f"xcd-contiguous {shape['xcd_contiguous']} ({shape['ms']:.2f} ms)")
+            f"xcd-contiguous {shape['xcd_contiguous']}, pipeline {shape['pipeline']} "
+            f"({shape['ms']:.2f} ms)")
+        for row in sorted(table, key=lambda r: r["ms"]):
+            log(f"  shape {row['block']:4d} x {row['unroll']:2d} xcd={int(row['xcd_contiguous'])} "
+                f"pipe={int(row['pipeline'])}: {row['ms']:.3f} ms")
 
     plain = None
     if not args.no_plain:
@@ -220,7 +224,8 @@ def main() -> int:
                             wl.observer_height, 0) as ef:
             ef.upload(wl.elevations.astype(np.float32))
             ef.compute_bands(0, 180)
-            ef.set_walk_shape(shape["block"], shape["unroll"], shape["xcd_contiguous"])
+            ef.set_walk_shape(shape["block"], shape["unroll"], shape["xcd_contiguous"],
+                              shape["pipeline"])
             f_ms, f_kernel_ms, _ = time_method(ef, "batched", args.steps, args.warmup)
         batched_f32 = {"value": point_steps / (f_ms / 1000.0), "ms_per_step": f_ms,
                        "kernel_ms": f_kernel_ms,
@@ -274,7 +279,8 @@ def main() -> int:
             "point_steps_per_step": point_steps,
             "steps_per_band": g.band_deltas_size,
             "real_steps_per_band": g.real_steps,
-            "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous")},
+            "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous",
+                                                 "pipeline")},
         },
         "roofline": {
             "bound": "hbm",

@@ -232,14 +232,17 @@ const char *tvs_status_string(tvs_status status);
 
 /* ---- launch shape and autotuning of the batched walk --------------------- */
 
-/* Points per work item (threads per block), steps unrolled per chunk, and
- * whether consecutive work items stay on one XCD; `ms` is a timing result.
+/* Points per work item (threads per block), steps unrolled per chunk,
+ * whether consecutive work items stay on one XCD, and whether the next
+ * chunk's heights are requested before the current chunk's arithmetic;
+ * `ms` is a timing result.
  * No reference counterpart: the Vulkan path's fixed 8x8x4 workgroup
  * (lib.rs:23, gpu.rs:10) is the only shape the reference has. */
 typedef struct tvs_walk_shape {
     uint32_t block;
     uint32_t unroll;
     uint32_t xcd_contiguous;
+    uint32_t pipeline;
     float ms;
 } tvs_walk_shape;
 

@@ -66,10 +66,10 @@ def test_batch_shapes_agree(tiles_and_truth):
         for i, t in enumerate(tiles):
             b.upload(i, t.elevations)
         b.compute_bands()
-        for block, unroll, xcd in [(64, 8, False), (128, 16, True), (512, 8, False),
-                                   (256, 4, True)]:
-            print(f"batch shape {block}x{unroll} xcd={xcd}", flush=True)
-            b.set_walk_shape(block, unroll, xcd)
+        for block, unroll, xcd, pipe in [(64, 8, False, False), (128, 16, True, False),
+                                         (512, 8, False, True), (256, 4, True, True)]:
+            print(f"batch shape {block}x{unroll} xcd={xcd} pipe={pipe}", flush=True)
+            b.set_walk_shape(block, unroll, xcd, pipe)
             b.reset_surfaces()
             b.run(True)
             for i in range(len(tiles)):
@@ -87,7 +87,7 @@ def test_autotune_keeps_results_exact(tiles_and_truth):
         before = e.surfaces()
         best, table = e.autotune(0, 30, reps=1)
         print("autotune", best, flush=True)
-        assert len(table) == 2 * 7 and best["ms"] > 0
+        assert len(table) == 2 * 11 and best["ms"] > 0
         np.testing.assert_array_equal(e.surfaces(), before)   # surfaces preserved
         assert e.walk_shape()["block"] == best["block"]
         e.reset_surfaces()

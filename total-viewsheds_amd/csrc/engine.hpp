@@ -87,11 +87,14 @@ struct WalkConfig {
     uint32_t block = 256;
     uint32_t unroll = 8;
     bool xcd_contiguous = true;
+    bool pipeline = false;    // next chunk's heights requested before this chunk's math
 };
 
-// Compiled (block, unroll) shapes of the tile walk.
-#define TVS_WALK_SHAPES(X) \
-    X(256, 8) X(128, 8) X(64, 8) X(512, 8) X(256, 4) X(256, 16) X(128, 16)
+// Compiled (block, unroll, pipeline) shapes of the tile walk.
+#define TVS_WALK_SHAPES(X)                                                 \
+    X(256, 8, false) X(128, 8, false) X(64, 8, false) X(512, 8, false)     \
+    X(256, 4, false) X(256, 16, false) X(128, 16, false)                   \
+    X(256, 8, true) X(128, 8, true) X(256, 4, true) X(512, 8, true)
 
 // Scratch for the device Axes::compute (keys, ids, radix-sort ping-pong,
 // histograms).

@@ -15,9 +15,9 @@ namespace {
 
 std::vector<tvs::WalkConfig> compiled_shapes() {
     std::vector<tvs::WalkConfig> out;
-#define TVS_SHAPE(B, U)                          \
-    out.push_back(tvs::WalkConfig{B, U, true});  \
-    out.push_back(tvs::WalkConfig{B, U, false});
+#define TVS_SHAPE(B, U, P)                          \
+    out.push_back(tvs::WalkConfig{B, U, true, P});  \
+    out.push_back(tvs::WalkConfig{B, U, false, P});
     TVS_WALK_SHAPES(TVS_SHAPE)
 #undef TVS_SHAPE
     return out;
@@ -34,7 +34,8 @@ extern "C" {
 
 tvs_status tvs_set_walk_shape(tvs_ctx *c, const tvs_walk_shape *shape) {
     if (!c || !shape) return TVS_ERR_INVALID_ARGUMENT;
-    tvs::WalkConfig cfg{shape->block, shape->unroll, shape->xcd_contiguous != 0};
+    tvs::WalkConfig cfg{shape->block, shape->unroll, shape->xcd_contiguous != 0,
+                        shape->pipeline != 0};
     if (!tvs::walk_shape_supported(cfg))
         return tfail(c, TVS_ERR_INVALID_ARGUMENT, "walk shape not compiled");
     c->walk_cfg = cfg;
@@ -46,6 +47,7 @@ tvs_status tvs_get_walk_shape(const tvs_ctx *c, tvs_walk_shape *out) {
     out->block = c->walk_cfg.block;
     out->unroll = c->walk_cfg.unroll;
     out->xcd_contiguous = c->walk_cfg.xcd_contiguous ? 1u : 0u;
+    out->pipeline = c->walk_cfg.pipeline ? 1u : 0u;
     out->ms = 0.0f;
     return TVS_OK;
 }
@@ -91,7 +93,8 @@ tvs_status tvs_autotune(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t rep
         if (st != TVS_OK) break;
         const float ms = total / static_cast<float>(reps);
         if (results && k < max_results)
-            results[k] = tvs_walk_shape{cfg.block, cfg.unroll, cfg.xcd_contiguous ? 1u : 0u, ms};
+            results[k] = tvs_walk_shape{cfg.block, cfg.unroll, cfg.xcd_contiguous ? 1u : 0u,
+                                        cfg.pipeline ? 1u : 0u, ms};
         ++k;
         if (ms < winner_ms) {
             winner_ms = ms;
@@ -110,7 +113,7 @@ tvs_status tvs_autotune(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t rep
     if (n_results) *n_results = k;
     if (best)
         *best = tvs_walk_shape{winner.block, winner.unroll, winner.xcd_contiguous ? 1u : 0u,
-                               winner_ms};
+                               winner.pipeline ? 1u : 0u, winner_ms};
     return TVS_OK;
 }
 

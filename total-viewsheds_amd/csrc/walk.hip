@@ -135,7 +135,7 @@ __device__ __forceinline__ TVS_GLOBAL T *global_ptr(T *p) {
 // One band per lane: (tile d, sector slot d.first_slot + rel, direction)
 // for point t.  FAST selects the split-reciprocal divide for the regular
 // steps (the tile's heights passed the host's range check).
-template <typename H, bool FAST, bool CAPTURE, int UNROLL>
+template <typename H, bool FAST, bool CAPTURE, int UNROLL, bool PIPE>
 __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool backward,
                                           uint32_t t, bool active) {
     const H *elev = static_cast<const H *>(d.elev);
@@ -191,22 +191,45 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
 
     uint32_t k = 0;
     const uint32_t fast_end = n_fast - n_fast % UNROLL;
-    for (; k < fast_end; k += UNROLL) {
-        float h[UNROLL];
-#pragma unroll
-        for (int u = 0; u < UNROLL; ++u) h[u] = load_height<H>(rsrc, pov_bytes, soff(off[k + u]));
+    auto chunk = [&](const float (&h)[UNROLL], uint32_t k0) {
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
             const float dh = h[u] - h0;
-            const float q = FAST ? div_by_reciprocal(dh, dist[k + u], rcp[k + u], rcp_lo[k + u])
-                                 : dh / dist[k + u];
-            const float angle = q - curv[k + u];
+            const float q = FAST ? div_by_reciprocal(dh, dist[k0 + u], rcp[k0 + u], rcp_lo[k0 + u])
+                                 : dh / dist[k0 + u];
+            const float angle = q - curv[k0 + u];
             const bool v = angle > max_angle;
-            if (CAPTURE && capture && v != visible) event(off[k + u]);
-            const float grown = surface + wt[k + u];
+            if (CAPTURE && capture && v != visible) event(off[k0 + u]);
+            const float grown = surface + wt[k0 + u];
             surface = v ? grown : surface;
             max_angle = v ? angle : max_angle;
             visible = v;
+        }
+    };
+    if (PIPE) {
+        // Heights of the next chunk are requested before this chunk's
+        // arithmetic; the last iteration re-reads its own chunk (in range).
+        if (fast_end) {
+            float h[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) h[u] = load_height<H>(rsrc, pov_bytes, soff(off[u]));
+            for (; k < fast_end; k += UNROLL) {
+                const uint32_t kn = k + UNROLL < fast_end ? k + UNROLL : k;
+                float hn[UNROLL];
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u)
+                    hn[u] = load_height<H>(rsrc, pov_bytes, soff(off[kn + u]));
+                chunk(h, k);
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u) h[u] = hn[u];
+            }
+        }
+    } else {
+        for (; k < fast_end; k += UNROLL) {
+            float h[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) h[u] = load_height<H>(rsrc, pov_bytes, soff(off[k + u]));
+            chunk(h, k);
         }
     }
     for (; k < tail_start; ++k) {
@@ -257,7 +280,7 @@ __device__ __forceinline__ uint32_t find_tile(const uint32_t *__restrict__ begin
     return lo;
 }
 
-template <typename H, bool CAPTURE, int BLOCK, int UNROLL>
+template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE>
 __global__ __launch_bounds__(BLOCK) void walk_tiles_kernel(
     const TileDesc *__restrict__ tiles, const uint32_t *__restrict__ item_begin,
     uint32_t n_tiles, uint32_t n_items, uint32_t items_per_xcd) {
@@ -276,9 +299,9 @@ __global__ __launch_bounds__(BLOCK) void walk_tiles_kernel(
     const bool active = t_raw < d.N;
     const uint32_t t = active ? t_raw : d.N - 1u;       // spare lanes shadow
     if (d.fast)
-        walk_band<H, true, CAPTURE, UNROLL>(d, pass >> 1, (pass & 1u) != 0, t, active);
+        walk_band<H, true, CAPTURE, UNROLL, PIPE>(d, pass >> 1, (pass & 1u) != 0, t, active);
     else
-        walk_band<H, false, CAPTURE, UNROLL>(d, pass >> 1, (pass & 1u) != 0, t, active);
+        walk_band<H, false, CAPTURE, UNROLL, PIPE>(d, pass >> 1, (pass & 1u) != 0, t, active);
 }
 
 // Per point, in sector order: cum += forward; cum += backward (the f32
@@ -321,12 +344,12 @@ __global__ void widen_i16_kernel(const int16_t *in, float *out, size_t n) {
         out[i] = static_cast<float>(in[i]);
 }
 
-template <typename H, bool CAPTURE, int BLOCK, int UNROLL>
+template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE>
 hipError_t launch_tiles_t(hipStream_t s, const TileDesc *tiles, const uint32_t *item_begin,
                           uint32_t n_tiles, uint32_t n_items, bool xcd_contiguous) {
     const uint32_t per_xcd = xcd_contiguous ? (n_items + 7u) / 8u : 0u;
     const uint32_t grid = xcd_contiguous ? per_xcd * 8u : n_items;
-    hipLaunchKernelGGL((walk_tiles_kernel<H, CAPTURE, BLOCK, UNROLL>), dim3(grid),
+    hipLaunchKernelGGL((walk_tiles_kernel<H, CAPTURE, BLOCK, UNROLL, PIPE>), dim3(grid),
                        dim3(BLOCK), 0, s, tiles, item_begin, n_tiles, n_items, per_xcd);
     return hipGetLastError();
 }
@@ -336,12 +359,12 @@ hipError_t launch_tiles_h(hipStream_t s, const TileDesc *tiles, const uint32_t *
                           uint32_t n_tiles, uint32_t n_items, bool capture,
                           const WalkConfig &cfg) {
     if (capture)   // ring capture is a checking path: one shape
-        return launch_tiles_t<H, true, 256, 8>(s, tiles, item_begin, n_tiles, n_items,
-                                               cfg.xcd_contiguous);
-#define TVS_SHAPE(B, U)                                                                   \
-    if (cfg.block == B && cfg.unroll == U)                                                \
-        return launch_tiles_t<H, false, B, U>(s, tiles, item_begin, n_tiles, n_items,     \
-                                              cfg.xcd_contiguous);
+        return launch_tiles_t<H, true, 256, 8, false>(s, tiles, item_begin, n_tiles, n_items,
+                                                      cfg.xcd_contiguous);
+#define TVS_SHAPE(B, U, P)                                                                \
+    if (cfg.block == B && cfg.unroll == U && cfg.pipeline == P)                           \
+        return launch_tiles_t<H, false, B, U, P>(s, tiles, item_begin, n_tiles, n_items,  \
+                                                 cfg.xcd_contiguous);
     TVS_WALK_SHAPES(TVS_SHAPE)
 #undef TVS_SHAPE
     return hipErrorInvalidValue;
@@ -363,7 +386,8 @@ hipError_t launch_walk_plain(tvs_ctx *c, uint32_t slot, uint32_t *ring_out) {
 }
 
 bool walk_shape_supported(const WalkConfig &cfg) {
-#define TVS_SHAPE(B, U) if (cfg.block == B && cfg.unroll == U) return true;
+#define TVS_SHAPE(B, U, P) \
+    if (cfg.block == B && cfg.unroll == U && cfg.pipeline == P) return true;
     TVS_WALK_SHAPES(TVS_SHAPE)
 #undef TVS_SHAPE
     return false;

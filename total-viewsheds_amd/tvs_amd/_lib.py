@@ -98,12 +98,14 @@ class WalkShape(ctypes.Structure):
         ("block", ctypes.c_uint32),
         ("unroll", ctypes.c_uint32),
         ("xcd_contiguous", ctypes.c_uint32),
+        ("pipeline", ctypes.c_uint32),
         ("ms", ctypes.c_float),
     ]
 
     def as_dict(self) -> dict:
         return {"block": self.block, "unroll": self.unroll,
-                "xcd_contiguous": bool(self.xcd_contiguous), "ms": self.ms}
+                "xcd_contiguous": bool(self.xcd_contiguous),
+                "pipeline": bool(self.pipeline), "ms": self.ms}
 
 
 class TvsError(RuntimeError):
@@ -357,9 +359,10 @@ class Engine:
         self._c(load().tvs_ring_overflow_count(self._h, ctypes.byref(v)))
         return int(v.value)
 
-    def set_walk_shape(self, block: int, unroll: int, xcd_contiguous: bool = True) -> None:
+    def set_walk_shape(self, block: int, unroll: int, xcd_contiguous: bool = True,
+                       pipeline: bool = False) -> None:
         self._c(load().tvs_set_walk_shape(self._h, ctypes.byref(
-            WalkShape(block, unroll, int(xcd_contiguous), 0.0))))
+            WalkShape(block, unroll, int(xcd_contiguous), int(pipeline), 0.0))))
 
     def walk_shape(self) -> dict:
         w = WalkShape()
@@ -444,9 +447,10 @@ class Batch:
     def compute_bands(self) -> None:
         self._c(load().tvs_batch_compute_bands(self._h))
 
-    def set_walk_shape(self, block: int, unroll: int, xcd_contiguous: bool = False) -> None:
+    def set_walk_shape(self, block: int, unroll: int, xcd_contiguous: bool = False,
+                       pipeline: bool = False) -> None:
         self._c(load().tvs_batch_set_walk_shape(self._h, ctypes.byref(
-            WalkShape(block, unroll, int(xcd_contiguous), 0.0))))
+            WalkShape(block, unroll, int(xcd_contiguous), int(pipeline), 0.0))))
 
     def run(self, use_graph: bool = True) -> None:
         self._c(load().tvs_batch_run(self._h, int(use_graph)))
