@@ -83,6 +83,64 @@ def cpu_baseline(wl, sectors: int, threads: int):
                        f"CPU loop, {threads} thread(s), {dt:.1f} s on {cpu}")}
 
 
+def measure_large(steps: int, warmup: int) -> dict:
+    """3601 x 3601 at 30 m, LoS 40 km (1333 steps per band), int16."""
+    import tvs_amd
+    wl = workload("large-3601")
+    log(f"large tile {wl.width}x{wl.width}: bands + tune + walk")
+    with tvs_amd.Engine(wl.width, wl.spacing, wl.max_line_of_sight, wl.rings_per_km,
+                        wl.observer_height, 0) as e:
+        g = e.geometry
+        e.upload(wl.elevations)
+        e.compute_bands(0, 180)
+        bands_ms = e.timing().bands_ms
+        shape, _ = e.autotune(0, 36, reps=1)
+        ms, kernel_ms, tm = time_method(e, "batched", steps, warmup)
+    ps = 180 * 2 * g.computable_points_count * g.band_deltas_size
+    log(f"large: {ms:.2f} ms/step, {ps / ms * 1e3:.3e} point-steps/s")
+    return {"workload": wl.name, "value": ps / (ms / 1000.0), "unit": "point-steps/s",
+            "ms_per_step": ms, "kernel_ms": kernel_ms, "reduce_ms": tm.reduce_ms,
+            "point_steps_per_step": ps, "bands_ms": bands_ms,
+            "achieved_GBps": ps * 2 / (kernel_ms / 1000.0) / 1e9,
+            "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous", "pipeline")}}
+
+
+def measure_tiles(steps: int, warmup: int) -> dict:
+    """64 uneven tiles (sides from {257, 401, 601, 801, 1201}, 30/90 m), all
+    180 sectors of all tiles in one walk launch + one reduction, replayed
+    from a hipGraph."""
+    import tvs_amd
+    from tvs_amd import demgen
+    tiles = demgen.tiles_64()
+    log(f"tiles-64: {sum(t.width * t.width for t in tiles)} DEM points in {len(tiles)} tiles")
+    with tvs_amd.Batch([(t.width, t.spacing, t.max_line_of_sight, t.rings_per_km,
+                         t.observer_height) for t in tiles]) as b:
+        for i, t in enumerate(tiles):
+            b.upload(i, t.elevations)
+        b.compute_bands()
+        bands_ms = b.timing().bands_ms
+        for _ in range(warmup):
+            b.run(True)
+        b.sync()
+        t0 = time.perf_counter()
+        graph_ms = []
+        for _ in range(steps):
+            b.run(True)
+            graph_ms.append(b.timing().walk_ms)
+        b.sync()
+        ms = (time.perf_counter() - t0) * 1000.0 / steps
+        ps = b.timing().point_steps
+        b.run(False)                         # the launch pair without the graph
+        tm = b.timing()
+    log(f"tiles-64: {ms:.2f} ms/step, {ps / ms * 1e3:.3e} point-steps/s")
+    return {"workload": "tiles-64", "value": ps / (ms / 1000.0), "unit": "point-steps/s",
+            "ms_per_step": ms, "graph_ms": float(np.mean(graph_ms)),
+            "kernel_ms": tm.walk_ms, "reduce_ms": tm.reduce_ms,
+            "point_steps_per_step": ps, "bands_ms": bands_ms,
+            "tiles": len(tiles),
+            "sides": sorted({t.width for t in tiles})}
+
+
 def copy_calibration(mib: int) -> int:
     """A plain device-to-device copy of known size (the runtime's copy
     kernel), to scale the FETCH_SIZE / WRITE_SIZE counters."""
@@ -156,6 +214,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default walk shape")
     ap.add_argument("--no-f32", action="store_true", help="skip the f32-heights batched run")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the large-3601 and tiles-64 figures")
     ap.add_argument("--cpu-sectors", type=int, default=18)
     ap.add_argument("--cpu-threads", type=int, default=1)
     ap.add_argument("--copy-calibration", type=int, metavar="MIB",
@@ -302,6 +362,9 @@ def main() -> int:
         "reduce_ms": tm.reduce_ms,
         "bands_ms": bands_ms,
     }
+    if not args.no_extra:
+        out["extra"] = [measure_large(max(2, args.steps // 2), 1),
+                        measure_tiles(max(2, args.steps // 2), 1)]
     if not args.no_cpu_baseline:
         log(f"CPU baseline: oracle walk over {args.cpu_sectors} sectors, "
             f"{args.cpu_threads} thread(s)")
