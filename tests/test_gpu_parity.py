@@ -270,3 +270,26 @@ def test_ring_overflow_is_counted_not_spilled(small):
         np.testing.assert_array_equal(e.surfaces(), cum)
         r = e.rings(0).reshape(-1, 3)
         assert np.all(r[:, 0] % 2 == 0)
+
+
+SHAPES = [(b, u, p) for (b, u, p) in [
+    (256, 8, False), (128, 8, False), (64, 8, False), (512, 8, False), (256, 4, False),
+    (256, 16, False), (128, 16, False), (256, 8, True), (128, 8, True), (256, 4, True),
+    (512, 8, True)]]
+
+
+@pytest.mark.parametrize("xcd", [True, False])
+def test_every_walk_shape_bit_exact(small, xcd):
+    """Each compiled launch shape of the batched walk gives the oracle's
+    surfaces (the tuner may pick any of them)."""
+    wl, dem, _, _, cum = small
+    with make_engine(wl) as e:
+        e.upload(wl.elevations)
+        e.compute_bands(0, 180)
+        for block, unroll, pipe in SHAPES:
+            note(f"shape {block}x{unroll} pipe={pipe} xcd={xcd}")
+            e.set_walk_shape(block, unroll, xcd, pipe)
+            e.reset_surfaces()
+            e.run(0, 180, "batched")
+            np.testing.assert_array_equal(e.surfaces(), cum,
+                                          err_msg=f"shape {block}x{unroll} pipe={pipe}")
