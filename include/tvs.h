@@ -215,6 +215,16 @@ tvs_status tvs_read_surfaces_f64(tvs_ctx *ctx, double *out, size_t n);
 tvs_status tvs_read_rings(tvs_ctx *ctx, uint32_t slot, uint32_t *out,
                           size_t n);
 
+/* One sector's ring events in compact form, for any number of events per
+ * band (no R limit): `counts[k]` = the count the reference stores at slot 0
+ * of band k (kernel.rs:243: 1 + events), `events` = every band's events
+ * after its PoV slot (kernel.rs:222-239 writes), band after band, k =
+ * 0..2N-1 (forward bands, then backward).  Call with events == NULL to get
+ * the counts and `*n_events` only.  Does not change the surfaces. */
+tvs_status tvs_compact_rings(tvs_ctx *ctx, uint32_t sector, uint32_t *counts,
+                             size_t n_counts, uint32_t *events,
+                             size_t events_capacity, uint64_t *n_events);
+
 /* Ring events dropped by the last run because a band had more than R-1. */
 tvs_status tvs_ring_overflow_count(tvs_ctx *ctx, uint64_t *out);
 

@@ -1,0 +1,62 @@
+"""Shape of the batched walk's inner loop in the gfx950 assembly (CPU only:
+hipcc cross-compiles).  Guards the properties the kernel's speed rests on:
+8 height loads with scalar offsets per chunk, the step tables read with
+scalar dwordx8 loads, and no divergent branch in the loop."""
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SRC = os.path.join(ROOT, "total-viewsheds_amd", "csrc", "walk.hip")
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+
+pytestmark = pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
+
+
+@pytest.fixture(scope="module")
+def asm(tmp_path_factory):
+    out = tmp_path_factory.mktemp("asm") / "walk.s"
+    subprocess.run([HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+                    "-fhip-fp32-correctly-rounded-divide-sqrt", "-Xclang", "-target-feature",
+                    "-Xclang", "-packed-fp32-ops", "--cuda-device-only", "-S", "-o", str(out),
+                    SRC], check=True, capture_output=True)
+    return out.read_text()
+
+
+def kernel_body(asm: str, mangled_part: str) -> str:
+    m = re.search(r"^(_Z\S*walk_tiles_kernel" + mangled_part + r"\S*):", asm, re.M)
+    assert m, f"kernel {mangled_part} not found"
+    start = m.end()
+    return asm[start:asm.index("s_endpgm", start)]
+
+
+def main_loop(body: str) -> str:
+    """The loop whose body holds the unrolled height loads."""
+    lines = body.splitlines()
+    heads = [i for i, l in enumerate(lines) if "Inner Loop Header" in l]
+    for h in heads:
+        end = next(i for i in range(h, len(lines)) if re.search(r"s_cbranch_scc\d", lines[i]))
+        chunk = "\n".join(lines[h:end + 1])
+        if len(re.findall(r"buffer_load_(s|u)short", chunk)) >= 8:
+            return chunk
+    raise AssertionError("no unrolled loop found")
+
+
+@pytest.mark.parametrize("pipe", ["Lb0E", "Lb1E"])
+def test_int16_walk_loop_shape(asm, pipe):
+    loop = main_loop(kernel_body(asm, "IsLb0ELi256ELi8E" + pipe))
+    assert "s_cbranch_execz" not in loop and "s_cbranch_vccz" not in loop
+    assert len(re.findall(r"buffer_load_(s|u)short\s+v\d+, v\d+, s\[\d+:\d+\], s\d+ offen", loop)) >= 8
+    assert "s_load_dwordx8" in loop
+    valu = len(re.findall(r"^\s+v_", loop, re.M))
+    assert valu <= 11 * 8 + 8, valu                # 11 per step (+ pipeline moves)
+    assert "scratch" not in loop and "buffer_store" not in loop
+
+
+def test_no_packed_fp32_and_no_spills(asm):
+    assert not re.search(r"\bv_pk_(add|mul|fma)_f32\b", asm)
+    sizes = re.findall(r"\.private_segment_fixed_size:\s+(\d+)", asm)
+    assert sizes and all(int(x) == 0 for x in sizes)

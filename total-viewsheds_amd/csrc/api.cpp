@@ -231,6 +231,10 @@ tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t cap
     d.R = c->rings_per_band;
     d.capture_slots = capture;
     d.fast = c->fast_div_ok ? 1u : 0u;
+    d.ring_mode = kRingRecords;
+    d.ring_counts = nullptr;
+    d.ring_offsets = nullptr;
+    d.ring_events = nullptr;
     *out = d;
     return TVS_OK;
 }
@@ -351,6 +355,9 @@ tvs_status tvs_destroy(tvs_ctx *c) {
     dev_free(c->ring);
     dev_free(c->overflow);
     dev_free(c->d_tile);
+    dev_free(c->d_ring_counts);
+    dev_free(c->d_ring_offsets);
+    dev_free(c->d_ring_events);
     dev_free(c->d_axes);
     dev_free(c->d_select_status);
     dev_free(c->d_begin);
@@ -624,6 +631,76 @@ tvs_status tvs_run_sector_with_band(tvs_ctx *c, const int32_t *deltas, const flo
     c->timing = tvs_timing{0.0f, 0.0f, c->timing.bands_ms, N ? 1u : 0u,
                            2ull * N * c->geo.band_deltas_size};
     if (cum_out) return tvs_read_surfaces(c, cum_out, n_out);
+    return TVS_OK;
+}
+
+// One sector's ring events in compact form: a counting walk, offsets scanned
+// on the host, then a writing walk.  Surfaces are not touched (the band
+// surfaces of these walks are not reduced).
+tvs_status tvs_compact_rings(tvs_ctx *c, uint32_t sector, uint32_t *counts, size_t n_counts,
+                             uint32_t *events, size_t events_capacity, uint64_t *n_events) {
+    if (!c) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null context");
+    if (sector >= tvs::kSectorSteps)
+        return fail(c, TVS_ERR_INVALID_ARGUMENT, "sector outside 0..180");
+    if (!c->has_f32) return fail(c, TVS_ERR_STATE, "no elevations uploaded");
+    if (!c->band_ready[sector]) return fail(c, TVS_ERR_STATE, "sector has no band data");
+    const uint32_t N = c->geo.computable_points_count;
+    const size_t bands = 2ull * N;
+    if (counts && n_counts != bands) return fail(c, TVS_ERR_SIZE, "counts length != 2N");
+    if (N == 0) {
+        if (n_events) *n_events = 0;
+        return TVS_OK;
+    }
+    if (!c->d_ring_counts) {
+        TVS_HIP(c, dev_alloc(&c->d_ring_counts, bands), "ring counts");
+        TVS_HIP(c, dev_alloc(&c->d_ring_offsets, bands), "ring offsets");
+    }
+    tvs::WalkConfig cfg;   // the capture path has one shape
+    tvs::TileDesc d;
+    tvs_status st = tvs::prepare_tile(c, sector, 1, 1, cfg, &d);
+    if (st != TVS_OK) return st;
+    const uint32_t n_items = d.blocks_per_pass * 2u;
+    const uint32_t begin[4] = {0u, n_items, 0u, N};
+    d.ring_mode = tvs::kRingCount;
+    d.ring_counts = c->d_ring_counts;
+    TVS_HIP(c, hipMemcpyAsync(c->d_tile, &d, sizeof d, hipMemcpyHostToDevice, c->stream), "descriptor");
+    TVS_HIP(c, hipMemcpyAsync(c->d_begin, begin, sizeof begin, hipMemcpyHostToDevice, c->stream), "descriptor");
+    TVS_HIP(c, tvs::launch_walk_tiles(c->stream, c->d_tile, c->d_begin, 1, n_items, c->has_i16,
+                                      true, cfg), "ring count walk");
+    std::vector<uint32_t> cnt(bands);
+    TVS_HIP(c, hipMemcpyAsync(cnt.data(), c->d_ring_counts, bands * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, c->stream), "ring counts");
+    TVS_HIP(c, hipStreamSynchronize(c->stream), "ring counts");
+    std::vector<uint64_t> offs(bands);
+    uint64_t total = 0;
+    for (size_t k = 0; k < bands; ++k) {
+        offs[k] = total;
+        total += cnt[k] - 1u;        // events after the PoV slot
+    }
+    if (counts) std::copy(cnt.begin(), cnt.end(), counts);
+    if (n_events) *n_events = total;
+    if (!events) return TVS_OK;
+    if (events_capacity < total)
+        return fail(c, TVS_ERR_SIZE, "events buffer holds " + std::to_string(events_capacity) +
+                                         ", sector has " + std::to_string(total));
+    if (c->ring_events_cap < total) {
+        dev_free(c->d_ring_events);
+        c->ring_events_cap = 0;
+        TVS_HIP(c, dev_alloc(&c->d_ring_events, total), "ring events");
+        c->ring_events_cap = total;
+    }
+    TVS_HIP(c, hipMemcpyAsync(c->d_ring_offsets, offs.data(), bands * sizeof(uint64_t),
+                              hipMemcpyHostToDevice, c->stream), "ring offsets");
+    d.ring_mode = tvs::kRingCompact;
+    d.ring_offsets = c->d_ring_offsets;
+    d.ring_events = c->d_ring_events;
+    TVS_HIP(c, hipMemcpyAsync(c->d_tile, &d, sizeof d, hipMemcpyHostToDevice, c->stream), "descriptor");
+    TVS_HIP(c, tvs::launch_walk_tiles(c->stream, c->d_tile, c->d_begin, 1, n_items, c->has_i16,
+                                      true, cfg), "ring write walk");
+    if (total)
+        TVS_HIP(c, hipMemcpyAsync(events, c->d_ring_events, total * sizeof(uint32_t),
+                                  hipMemcpyDeviceToHost, c->stream), "ring events");
+    TVS_HIP(c, hipStreamSynchronize(c->stream), "ring events");
     return TVS_OK;
 }
 

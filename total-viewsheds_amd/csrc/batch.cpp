@@ -37,6 +37,7 @@ struct tvs_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     tvs_timing timing{};
+    bool ran = false;            // events recorded by a run
     std::string err;
 };
 
@@ -262,6 +263,7 @@ tvs_status tvs_batch_run(tvs_batch *b, uint32_t use_graph) {
     for (tvs_ctx *c : b->tiles)
         steps += 180ull * 2ull * c->geo.computable_points_count * c->geo.band_deltas_size;
     b->timing = tvs_timing{0.0f, 0.0f, b->timing.bands_ms, 1u, steps};
+    b->ran = true;
     if (use_graph) {
         if (!b->exec) {
             TVS_BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "capture");
@@ -326,6 +328,10 @@ tvs_status tvs_batch_read_surfaces_f64(tvs_batch *b, uint32_t tile, double *out,
 
 tvs_status tvs_batch_get_timing(tvs_batch *b, tvs_timing *out) {
     if (!b || !out) return bfail(b, TVS_ERR_INVALID_ARGUMENT, "null argument");
+    if (!b->ran) {
+        *out = b->timing;
+        return TVS_OK;
+    }
     TVS_BHIP(b, hipEventSynchronize(b->ev[2]), "event");
     float ms = 0.0f;
     TVS_BHIP(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]), "event time");

@@ -79,7 +79,16 @@ struct TileDesc {
     uint32_t blocks_per_pass;    // ceil(N / block)
     uint32_t R, capture_slots;
     uint32_t fast;               // heights passed the fast-divide range check
+    uint32_t ring_mode;          // kRingRecords / kRingCount / kRingCompact
+    uint32_t *ring_counts;       // u32[2N]: ring_id per band (kRingCount)
+    const uint64_t *ring_offsets;  // u64[2N]: first event of each band (kRingCompact)
+    uint32_t *ring_events;       // packed event ids (kRingCompact)
 };
+
+// Ring capture modes of the tile walk.
+constexpr uint32_t kRingRecords = 0;
+constexpr uint32_t kRingCount = 1;
+constexpr uint32_t kRingCompact = 2;
 
 // Launch shape of the tile walk: points per work item (= threads per
 // block), steps unrolled per chunk, XCD-contiguous item mapping.
@@ -131,6 +140,10 @@ struct tvs_ctx {
     std::vector<tvs::SectorMeta> meta_host;
 
     tvs::WalkConfig walk_cfg;
+    uint32_t *d_ring_counts = nullptr;    // compact ring export
+    uint64_t *d_ring_offsets = nullptr;
+    uint32_t *d_ring_events = nullptr;
+    size_t ring_events_cap = 0;
     tvs::TileDesc *d_tile = nullptr;     // device copy of this DEM's descriptor
     uint32_t *d_begin = nullptr;         // {0, items} and {0, N}
     float *cum = nullptr;      // f32[N]  cumulative surfaces

@@ -116,6 +116,14 @@ __device__ __forceinline__ float load_height<float>(__amdgpu_buffer_rsrc_t rsrc,
         __builtin_amdgcn_raw_buffer_load_b32(rsrc, pov_bytes, soff_bytes, 0));
 }
 
+// surface + w when v, else surface: fma(1, w, s) = RN(s + w) and
+// fma(0, w, s) = s for the finite w >= +0 and s != -0 of the walk
+// (kernel.rs:216).  Written as one unconditional FMA so the compiler keeps
+// the step branch-free instead of sinking the weight's load into a branch.
+__device__ __forceinline__ float add_if(bool v, float surface, float w) {
+    return __builtin_fmaf(v ? 1.0f : 0.0f, w, surface);
+}
+
 // Buffer resource word 3 for raw (untyped) loads on gfx9-family parts.
 constexpr int kBufferFlags = 0x00020000;
 
@@ -172,21 +180,31 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
 
     // Ring records (kernel.rs:141-243): an event is written exactly when the
     // visibility changes (opening and closing are exclusive), plus a final
-    // close if the last step is visible.
+    // close if the last step is visible.  Modes: fixed R-slot records
+    // (kRingRecords), per-band event count only (kRingCount), or the events
+    // packed at per-band offsets (kRingCompact).
     const bool capture = CAPTURE && active && rel < d.capture_slots;
+    const uint32_t kernel_id = backward ? t + d.N : t;
     TVS_GLOBAL uint32_t *band_ring = nullptr;
     uint32_t ring_id = 1;
     unsigned dropped = 0;
     if (CAPTURE && capture) {
-        const uint32_t kernel_id = backward ? t + d.N : t;
-        band_ring = global_ptr(d.ring) + static_cast<size_t>(rel) * 2u * d.N * d.R +
-                    static_cast<size_t>(kernel_id) * d.R;
-        if (1u < d.R) band_ring[1] = pov; else ++dropped;
+        if (d.ring_mode == kRingRecords) {
+            band_ring = global_ptr(d.ring) + static_cast<size_t>(rel) * 2u * d.N * d.R +
+                        static_cast<size_t>(kernel_id) * d.R;
+            if (1u < d.R) band_ring[1] = pov; else ++dropped;
+        } else if (d.ring_mode == kRingCompact) {
+            band_ring = global_ptr(d.ring_events) + global_ptr(d.ring_offsets)[kernel_id];
+        }
     }
     auto event = [&](int32_t o) {
         ++ring_id;
-        if (ring_id < d.R) band_ring[ring_id] = pov + static_cast<uint32_t>(sgn * o);
-        else ++dropped;
+        const uint32_t id = pov + static_cast<uint32_t>(sgn * o);
+        if (d.ring_mode == kRingRecords) {
+            if (ring_id < d.R) band_ring[ring_id] = id; else ++dropped;
+        } else if (d.ring_mode == kRingCompact) {
+            band_ring[ring_id - 2u] = id;
+        }
     };
 
     uint32_t k = 0;
@@ -200,8 +218,7 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
             const float angle = q - curv[k0 + u];
             const bool v = angle > max_angle;
             if (CAPTURE && capture && v != visible) event(off[k0 + u]);
-            const float grown = surface + wt[k0 + u];
-            surface = v ? grown : surface;
+            surface = add_if(v, surface, wt[k0 + u]);
             max_angle = v ? angle : max_angle;
             visible = v;
         }
@@ -241,8 +258,7 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
         const float angle = q - curv[k];
         const bool v = angle > max_angle;
         if (CAPTURE && capture && v != visible) event(o);
-        const float grown = surface + wt[k];
-        surface = v ? grown : surface;
+        surface = add_if(v, surface, wt[k]);
         max_angle = v ? angle : max_angle;
         visible = v;
     }
@@ -262,8 +278,12 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
     }
     if (CAPTURE && capture) {
         if (visible) event(d.n_steps ? off[d.n_steps - 1] : 0);
-        if (d.R > 0) band_ring[0] = ring_id;
-        if (dropped) atomicAdd(d.overflow, static_cast<unsigned long long>(dropped));
+        if (d.ring_mode == kRingRecords) {
+            if (d.R > 0) band_ring[0] = ring_id;
+            if (dropped) atomicAdd(d.overflow, static_cast<unsigned long long>(dropped));
+        } else if (d.ring_mode == kRingCount) {
+            global_ptr(d.ring_counts)[kernel_id] = ring_id;
+        }
     }
     const uint32_t pass = 2u * rel + (backward ? 1u : 0u);
     if (active) global_ptr(d.partial)[static_cast<size_t>(pass) * d.N + t] = surface;

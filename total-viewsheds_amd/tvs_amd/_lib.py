@@ -162,6 +162,8 @@ def load() -> ctypes.CDLL:
         "tvs_band_cache_save": (st, [vp, ctypes.c_char_p, ctypes.c_uint32]),
         "tvs_band_cache_load": (st, [vp, ctypes.c_char_p, ctypes.c_uint32]),
         "tvs_io_last_error": (ctypes.c_char_p, []),
+        "tvs_compact_rings": (st, [vp, ctypes.c_uint32, vp, ctypes.c_size_t, vp,
+                                   ctypes.c_size_t, P(ctypes.c_uint64)]),
         "tvs_set_band_method": (st, [vp, ctypes.c_int]),
         "tvs_bands_sorted_count": (st, [vp, P(ctypes.c_uint32)]),
         "tvs_set_walk_shape": (st, [vp, P(WalkShape)]),
@@ -353,6 +355,19 @@ class Engine:
         out = np.empty(max(n, 1), np.uint32)
         self._c(load().tvs_read_rings(self._h, slot, _ptr(out), n))
         return out[:n]
+
+    def compact_rings(self, sector: int):
+        """(counts u32[2N], events u32[total]) of one sector: counts[k] is the
+        count slot of band k, events its ids after the PoV slot."""
+        bands = 2 * self.n_points
+        counts = np.empty(max(bands, 1), np.uint32)
+        total = ctypes.c_uint64(0)
+        self._c(load().tvs_compact_rings(self._h, sector, _ptr(counts), bands, None, 0,
+                                         ctypes.byref(total)))
+        events = np.empty(max(int(total.value), 1), np.uint32)
+        self._c(load().tvs_compact_rings(self._h, sector, _ptr(counts), bands, _ptr(events),
+                                         events.size, ctypes.byref(total)))
+        return counts[:bands], events[:int(total.value)]
 
     def ring_overflow(self) -> int:
         v = ctypes.c_uint64(0)
