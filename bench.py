@@ -102,7 +102,8 @@ def measure_large(steps: int, warmup: int) -> dict:
             "ms_per_step": ms, "kernel_ms": kernel_ms, "reduce_ms": tm.reduce_ms,
             "point_steps_per_step": ps, "bands_ms": bands_ms,
             "achieved_GBps": ps * 2 / (kernel_ms / 1000.0) / 1e9,
-            "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous", "pipeline")}}
+            "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous", "pipeline",
+                                                 "persistent")}}
 
 
 def measure_tiles(steps: int, warmup: int) -> dict:
@@ -256,11 +257,12 @@ def main() -> int:
     if not args.no_tune:
         shape, table = e.autotune(0, 180, reps=1)
         log(f"autotuned walk shape: block {shape['block']}, unroll {shape['unroll']}, "
-            f"xcd-contiguous {shape['xcd_contiguous']}, pipeline {shape['pipeline']} "
+            f"xcd-contiguous {shape['xcd_contiguous']}, pipeline {shape['pipeline']}, "
+            f"persistent {shape['persistent']} "
             f"({shape['ms']:.2f} ms)")
         for row in sorted(table, key=lambda r: r["ms"]):
             log(f"  shape {row['block']:4d} x {row['unroll']:2d} xcd={int(row['xcd_contiguous'])} "
-                f"pipe={int(row['pipeline'])}: {row['ms']:.3f} ms")
+                f"pipe={int(row['pipeline'])} pers={int(row['persistent'])}: {row['ms']:.3f} ms")
 
     plain = None
     if not args.no_plain:
@@ -285,7 +287,7 @@ def main() -> int:
             ef.upload(wl.elevations.astype(np.float32))
             ef.compute_bands(0, 180)
             ef.set_walk_shape(shape["block"], shape["unroll"], shape["xcd_contiguous"],
-                              shape["pipeline"])
+                              shape["pipeline"], shape["persistent"])
             f_ms, f_kernel_ms, _ = time_method(ef, "batched", args.steps, args.warmup)
         batched_f32 = {"value": point_steps / (f_ms / 1000.0), "ms_per_step": f_ms,
                        "kernel_ms": f_kernel_ms,
@@ -340,7 +342,7 @@ def main() -> int:
             "steps_per_band": g.band_deltas_size,
             "real_steps_per_band": g.real_steps,
             "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous",
-                                                 "pipeline")},
+                                                 "pipeline", "persistent")},
         },
         "roofline": {
             "bound": "hbm",

@@ -244,8 +244,9 @@ const char *tvs_status_string(tvs_status status);
 
 /* Points per work item (threads per block), steps unrolled per chunk,
  * whether consecutive work items stay on one XCD, and whether the next
- * chunk's heights are requested before the current chunk's arithmetic;
- * `ms` is a timing result.
+ * chunk's heights are requested before the current chunk's arithmetic, and
+ * whether a grid of a few blocks per CU strides over all work items
+ * (`persistent`, which ignores `xcd_contiguous`); `ms` is a timing result.
  * No reference counterpart: the Vulkan path's fixed 8x8x4 workgroup
  * (lib.rs:23, gpu.rs:10) is the only shape the reference has. */
 typedef struct tvs_walk_shape {
@@ -253,13 +254,15 @@ typedef struct tvs_walk_shape {
     uint32_t unroll;
     uint32_t xcd_contiguous;
     uint32_t pipeline;
+    uint32_t persistent;
     float ms;
 } tvs_walk_shape;
 
 tvs_status tvs_set_walk_shape(tvs_ctx *ctx, const tvs_walk_shape *shape);
 tvs_status tvs_get_walk_shape(const tvs_ctx *ctx, tvs_walk_shape *out);
 
-/* Number of shapes compiled into the library (x2 item mappings). */
+/* Number of shapes the tuner times: each compiled shape in both item
+ * mappings, persistent shapes once. */
 size_t tvs_walk_shape_count(void);
 
 /* Time every compiled shape on the context's own DEM over sectors

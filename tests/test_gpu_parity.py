@@ -272,10 +272,12 @@ def test_ring_overflow_is_counted_not_spilled(small):
         assert np.all(r[:, 0] % 2 == 0)
 
 
-SHAPES = [(b, u, p) for (b, u, p) in [
-    (256, 8, False), (128, 8, False), (64, 8, False), (512, 8, False), (256, 4, False),
-    (256, 16, False), (128, 16, False), (256, 8, True), (128, 8, True), (256, 4, True),
-    (512, 8, True)]]
+SHAPES = [  # (block, unroll, pipeline, persistent)
+    (256, 8, False, False), (128, 8, False, False), (64, 8, False, False),
+    (512, 8, False, False), (256, 4, False, False), (256, 16, False, False),
+    (128, 16, False, False), (256, 8, True, False), (128, 8, True, False),
+    (256, 4, True, False), (512, 8, True, False), (256, 8, False, True),
+    (128, 8, False, True), (64, 8, False, True)]
 
 
 @pytest.mark.parametrize("xcd", [True, False])
@@ -286,10 +288,11 @@ def test_every_walk_shape_bit_exact(small, xcd):
     with make_engine(wl) as e:
         e.upload(wl.elevations)
         e.compute_bands(0, 180)
-        for block, unroll, pipe in SHAPES:
-            note(f"shape {block}x{unroll} pipe={pipe} xcd={xcd}")
-            e.set_walk_shape(block, unroll, xcd, pipe)
+        for block, unroll, pipe, pers in SHAPES:
+            note(f"shape {block}x{unroll} pipe={pipe} pers={pers} xcd={xcd}")
+            e.set_walk_shape(block, unroll, xcd, pipe, pers)
             e.reset_surfaces()
             e.run(0, 180, "batched")
             np.testing.assert_array_equal(e.surfaces(), cum,
-                                          err_msg=f"shape {block}x{unroll} pipe={pipe}")
+                                          err_msg=f"shape {block}x{unroll} pipe={pipe} "
+                                                  f"pers={pers}")

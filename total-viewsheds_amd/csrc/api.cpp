@@ -199,13 +199,12 @@ tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t cap
                 "band surfaces");
         c->partial_cap = count;
     }
-    int32_t bias = 0;
-    for (uint32_t s = first; s < first + count; ++s)
-        bias = std::max(bias, std::max(c->meta_host[s].max_off, -c->meta_host[s].min_off));
     TileDesc d{};
     d.elev = c->has_i16 ? static_cast<const void *>(c->elev_i16)
                         : static_cast<const void *>(c->elev_f32);
     d.off = c->tab.off;
+    d.pos_f = c->tab.pos_f;
+    d.pos_b = c->tab.pos_b;
     d.dist = c->tab.dist;
     d.rcp = c->tab.rcp;
     d.rcp_lo = c->tab.rcp_lo;
@@ -226,7 +225,6 @@ tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t cap
     d.L = c->geo.max_los_as_points;
     d.w = c->geo.dem_width;
     d.obs = c->params.observer_height;
-    d.bias = bias;
     d.blocks_per_pass = (N + cfg.block - 1) / cfg.block;
     d.R = c->rings_per_band;
     d.capture_slots = capture;
@@ -316,6 +314,8 @@ tvs_status tvs_create(const tvs_params *p, tvs_ctx **out) {
     if (e == hipSuccess) e = dev_alloc(&c->tab.delta, tab_n);
     if (e == hipSuccess) e = dev_alloc(&c->tab.dist, tab_n);
     if (e == hipSuccess) e = dev_alloc(&c->tab.off, tab_n);
+    if (e == hipSuccess) e = dev_alloc(&c->tab.pos_f, tab_n);
+    if (e == hipSuccess) e = dev_alloc(&c->tab.pos_b, tab_n);
     if (e == hipSuccess) e = dev_alloc(&c->tab.rcp, tab_n);
     if (e == hipSuccess) e = dev_alloc(&c->tab.rcp_lo, tab_n);
     if (e == hipSuccess) e = dev_alloc(&c->tab.curv, tab_n);
@@ -344,6 +344,8 @@ tvs_status tvs_destroy(tvs_ctx *c) {
     dev_free(c->tab.delta);
     dev_free(c->tab.dist);
     dev_free(c->tab.off);
+    dev_free(c->tab.pos_f);
+    dev_free(c->tab.pos_b);
     dev_free(c->tab.rcp);
     dev_free(c->tab.rcp_lo);
     dev_free(c->tab.curv);

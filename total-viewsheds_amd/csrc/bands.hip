@@ -280,20 +280,29 @@ __global__ __launch_bounds__(kScanThreads) void step_table_kernel(
         red_tail[tid >> 6] = last_nonzero;
     }
     __syncthreads();
+    // Block-wide results (every thread needs the bias).
+    int32_t lo = 0, hi = 0;   // the PoV itself is on every band
+    uint32_t fast = n_steps, tail = 0;
+    for (uint32_t wv = 0; wv < kScanThreads / 64; ++wv) {
+        lo = min(lo, red_min[wv]);
+        hi = max(hi, red_max[wv]);
+        fast = min(fast, red_fast[wv]);
+        tail = max(tail, red_tail[wv]);
+    }
+    const int32_t bias = max(hi, -lo);
+    for (uint32_t i = begin; i < end; ++i) {
+        const int32_t o = tab.off[tb + i];
+        tab.pos_f[tb + i] = static_cast<uint32_t>(bias + o);
+        tab.pos_b[tb + i] = static_cast<uint32_t>(bias - o);
+    }
     if (tid == 0) {
         SectorMeta m{};
-        m.min_off = 0;   // the PoV itself is on every band
-        m.max_off = 0;
-        m.n_fast = n_steps;
-        m.tail_start = 0;
-        for (uint32_t wv = 0; wv < kScanThreads / 64; ++wv) {
-            m.min_off = min(m.min_off, red_min[wv]);
-            m.max_off = max(m.max_off, red_max[wv]);
-            m.n_fast = min(m.n_fast, red_fast[wv]);
-            m.tail_start = max(m.tail_start, red_tail[wv]);
-        }
-        if (m.n_fast > m.tail_start) m.n_fast = m.tail_start;
+        m.min_off = lo;
+        m.max_off = hi;
+        m.n_fast = fast > tail ? tail : fast;
+        m.tail_start = tail;
         m.status = 0;
+        m.bias = static_cast<uint32_t>(bias);
         tab.meta[slot] = m;
     }
 }

@@ -98,6 +98,8 @@ tvs_status prepare(tvs_batch *b) {
         tvs_status st = tvs::prepare_tile(c, 0, tvs::kSectorSteps, 0, b->cfg, &d);
         if (st != TVS_OK) return from_tile(b, i, st);
         d.off = o->tab.off;
+        d.pos_f = o->tab.pos_f;
+        d.pos_b = o->tab.pos_b;
         d.dist = o->tab.dist;
         d.rcp = o->tab.rcp;
         d.rcp_lo = o->tab.rcp_lo;
@@ -247,7 +249,7 @@ tvs_status tvs_batch_compute_bands(tvs_batch *b) {
 tvs_status tvs_batch_set_walk_shape(tvs_batch *b, const tvs_walk_shape *shape) {
     if (!b || !shape) return bfail(b, TVS_ERR_INVALID_ARGUMENT, "null argument");
     tvs::WalkConfig cfg{shape->block, shape->unroll, shape->xcd_contiguous != 0,
-                        shape->pipeline != 0};
+                        shape->pipeline != 0, shape->persistent != 0};
     if (!tvs::walk_shape_supported(cfg))
         return bfail(b, TVS_ERR_INVALID_ARGUMENT, "walk shape not compiled");
     b->cfg = cfg;

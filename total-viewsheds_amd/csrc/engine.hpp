@@ -36,7 +36,8 @@ struct SectorMeta {
     uint32_t n_fast;
     uint32_t tail_start;
     uint32_t status;     // non-zero: band compile hit a reference panic site
-    uint32_t pad[3];
+    uint32_t bias;       // max |cumulative offset|: positions below are bias +/- off
+    uint32_t pad[2];
 };
 
 // Device-resident per-sector step tables (structure of arrays, `stride`
@@ -47,6 +48,8 @@ struct StepTables {
     int32_t *delta = nullptr;  // band_deltas            (band_of_sight.rs:165)
     float *dist = nullptr;     // band_distances         (band_of_sight.rs:168)
     int32_t *off = nullptr;    // inclusive prefix sum of delta
+    uint32_t *pos_f = nullptr; // meta.bias + off: forward element position
+    uint32_t *pos_b = nullptr; // meta.bias - off: backward element position
     float *rcp = nullptr;      // RN(1 / dist)
     float *rcp_lo = nullptr;   // RN((1 - dist * rcp) * rcp): 1/dist - rcp
     float *curv = nullptr;     // RN(dist / 12742000)    (kernel.rs:175)
@@ -61,6 +64,8 @@ struct StepTables {
 struct TileDesc {
     const void *elev;            // int16 or f32, w*w row-major
     const int32_t *off;          // step tables, slot 0 (see StepTables)
+    const uint32_t *pos_f;
+    const uint32_t *pos_b;
     const float *dist;
     const float *rcp;
     const float *rcp_lo;
@@ -75,7 +80,6 @@ struct TileDesc {
     uint32_t stride, n_steps, first_slot, count;
     uint32_t N, T, L, w;
     float obs;
-    int32_t bias;                // >= every |cumulative offset| of the slots
     uint32_t blocks_per_pass;    // ceil(N / block)
     uint32_t R, capture_slots;
     uint32_t fast;               // heights passed the fast-divide range check
@@ -97,13 +101,16 @@ struct WalkConfig {
     uint32_t unroll = 8;
     bool xcd_contiguous = true;
     bool pipeline = false;    // next chunk's heights requested before this chunk's math
+    bool persistent = false;  // a few blocks per CU stride over all items
 };
 
-// Compiled (block, unroll, pipeline) shapes of the tile walk.
-#define TVS_WALK_SHAPES(X)                                                 \
-    X(256, 8, false) X(128, 8, false) X(64, 8, false) X(512, 8, false)     \
-    X(256, 4, false) X(256, 16, false) X(128, 16, false)                   \
-    X(256, 8, true) X(128, 8, true) X(256, 4, true) X(512, 8, true)
+// Compiled (block, unroll, pipeline, persistent) shapes of the tile walk.
+#define TVS_WALK_SHAPES(X)                                                               \
+    X(256, 8, false, false) X(128, 8, false, false) X(64, 8, false, false)               \
+    X(512, 8, false, false) X(256, 4, false, false) X(256, 16, false, false)             \
+    X(128, 16, false, false) X(256, 8, true, false) X(128, 8, true, false)               \
+    X(256, 4, true, false) X(512, 8, true, false) X(256, 8, false, true)                 \
+    X(128, 8, false, true) X(64, 8, false, true)
 
 // Scratch for the device Axes::compute (keys, ids, radix-sort ping-pong,
 // histograms).

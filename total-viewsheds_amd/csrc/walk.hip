@@ -151,25 +151,23 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
     const uint32_t pov = (t / d.T + d.L) * d.w + (t % d.T + d.L);
     const uint32_t pov_bytes = pov * static_cast<uint32_t>(sizeof(H));
     const float h0 = static_cast<float>(global_ptr(elev)[pov]) + d.obs;
+    const uint32_t n_fast = global_ptr(d.meta)[slot].n_fast;
+    const uint32_t tail_start = global_ptr(d.meta)[slot].tail_start;
+    const uint32_t bias = global_ptr(d.meta)[slot].bias;
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<H *>(elev - d.bias), 0,
-        static_cast<int>((static_cast<uint32_t>(d.bias) + d.w * d.w) * sizeof(H)),
-        kBufferFlags);
-    const int32_t sgn = backward ? -1 : 1;
-    // Scalar byte offset of a step: (sgn * off + bias) * sizeof(H) >= 0.
-    auto soff = [&](int32_t o) -> uint32_t {
-        return static_cast<uint32_t>(sgn * o + d.bias) * static_cast<uint32_t>(sizeof(H));
-    };
+        const_cast<H *>(elev - bias), 0,
+        static_cast<int>((bias + d.w * d.w) * sizeof(H)), kBufferFlags);
 
     const size_t tb = static_cast<size_t>(slot) * d.stride;
-    const TVS_GLOBAL int32_t *__restrict__ off = global_ptr(d.off + tb);
+    // Element position of each step relative to the resource base:
+    // bias + off forward, bias - off backward (>= 0, from the step table).
+    const TVS_GLOBAL uint32_t *__restrict__ pos = global_ptr((backward ? d.pos_b : d.pos_f) + tb);
+    auto soff = [&](uint32_t p) -> uint32_t { return p * static_cast<uint32_t>(sizeof(H)); };
     const TVS_GLOBAL float *__restrict__ dist = global_ptr(d.dist + tb);
     const TVS_GLOBAL float *__restrict__ rcp = global_ptr(d.rcp + tb);
     const TVS_GLOBAL float *__restrict__ rcp_lo = global_ptr(d.rcp_lo + tb);
     const TVS_GLOBAL float *__restrict__ curv = global_ptr(d.curv + tb);
     const TVS_GLOBAL float *__restrict__ wt = global_ptr(d.wt + tb);
-    const uint32_t n_fast = global_ptr(d.meta)[slot].n_fast;
-    const uint32_t tail_start = global_ptr(d.meta)[slot].tail_start;
 
     // `max_angle = v ? angle : max_angle` is f32::max(angle, max_angle)
     // (kernel.rs:233) for every input the walk meets: v is `angle >
@@ -197,9 +195,9 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
             band_ring = global_ptr(d.ring_events) + global_ptr(d.ring_offsets)[kernel_id];
         }
     }
-    auto event = [&](int32_t o) {
+    auto event = [&](uint32_t p) {
         ++ring_id;
-        const uint32_t id = pov + static_cast<uint32_t>(sgn * o);
+        const uint32_t id = pov + p - bias;   // pov +/- off
         if (d.ring_mode == kRingRecords) {
             if (ring_id < d.R) band_ring[ring_id] = id; else ++dropped;
         } else if (d.ring_mode == kRingCompact) {
@@ -217,7 +215,7 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
                                  : dh / dist[k0 + u];
             const float angle = q - curv[k0 + u];
             const bool v = angle > max_angle;
-            if (CAPTURE && capture && v != visible) event(off[k0 + u]);
+            if (CAPTURE && capture && v != visible) event(pos[k0 + u]);
             surface = add_if(v, surface, wt[k0 + u]);
             max_angle = v ? angle : max_angle;
             visible = v;
@@ -229,13 +227,13 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
         if (fast_end) {
             float h[UNROLL];
 #pragma unroll
-            for (int u = 0; u < UNROLL; ++u) h[u] = load_height<H>(rsrc, pov_bytes, soff(off[u]));
+            for (int u = 0; u < UNROLL; ++u) h[u] = load_height<H>(rsrc, pov_bytes, soff(pos[u]));
             for (; k < fast_end; k += UNROLL) {
                 const uint32_t kn = k + UNROLL < fast_end ? k + UNROLL : k;
                 float hn[UNROLL];
 #pragma unroll
                 for (int u = 0; u < UNROLL; ++u)
-                    hn[u] = load_height<H>(rsrc, pov_bytes, soff(off[kn + u]));
+                    hn[u] = load_height<H>(rsrc, pov_bytes, soff(pos[kn + u]));
                 chunk(h, k);
 #pragma unroll
                 for (int u = 0; u < UNROLL; ++u) h[u] = hn[u];
@@ -245,12 +243,12 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
         for (; k < fast_end; k += UNROLL) {
             float h[UNROLL];
 #pragma unroll
-            for (int u = 0; u < UNROLL; ++u) h[u] = load_height<H>(rsrc, pov_bytes, soff(off[k + u]));
+            for (int u = 0; u < UNROLL; ++u) h[u] = load_height<H>(rsrc, pov_bytes, soff(pos[k + u]));
             chunk(h, k);
         }
     }
     for (; k < tail_start; ++k) {
-        const int32_t o = off[k];
+        const uint32_t o = pos[k];
         const float dk = dist[k];
         const float dh = load_height<H>(rsrc, pov_bytes, soff(o)) - h0;
         const float q = (FAST && k < n_fast) ? div_by_reciprocal(dh, dk, rcp[k], rcp_lo[k])
@@ -266,9 +264,10 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
         // Zero tail: delta 0 keeps the point, distance +0 gives the same
         // angle (dh/0 - 0: +inf, -inf or NaN) at every step, and the weight
         // 0 * 0.0174533 leaves the surface unchanged.
-        const int32_t o = off[k];
+        const uint32_t o = pos[k];
         const float dh = load_height<H>(rsrc, pov_bytes, soff(o)) - h0;
         const float angle = dh / dist[k] - curv[k];
+#pragma unroll 8
         for (; k < d.n_steps; ++k) {
             const bool v = angle > max_angle;
             if (CAPTURE && capture && v != visible) event(o);
@@ -277,7 +276,7 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
         }
     }
     if (CAPTURE && capture) {
-        if (visible) event(d.n_steps ? off[d.n_steps - 1] : 0);
+        if (visible) event(d.n_steps ? pos[d.n_steps - 1] : bias);
         if (d.ring_mode == kRingRecords) {
             if (d.R > 0) band_ring[0] = ring_id;
             if (dropped) atomicAdd(d.overflow, static_cast<unsigned long long>(dropped));
@@ -301,15 +300,9 @@ __device__ __forceinline__ uint32_t find_tile(const uint32_t *__restrict__ begin
 }
 
 template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE>
-__global__ __launch_bounds__(BLOCK) void walk_tiles_kernel(
-    const TileDesc *__restrict__ tiles, const uint32_t *__restrict__ item_begin,
-    uint32_t n_tiles, uint32_t n_items, uint32_t items_per_xcd) {
-    // With items_per_xcd != 0, consecutive block ids (dispatched to the 8
-    // XCDs round-robin) are remapped so each XCD takes a contiguous run of
-    // items: one DEM region per L2.
-    const uint32_t b = blockIdx.x;
-    const uint32_t item = items_per_xcd ? (b & 7u) * items_per_xcd + (b >> 3) : b;
-    if (item >= n_items) return;
+__device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
+                                          const uint32_t *__restrict__ item_begin,
+                                          uint32_t n_tiles, uint32_t item) {
     const uint32_t ti = n_tiles > 1 ? find_tile(item_begin, n_tiles, item) : 0u;
     const TileDesc d = tiles[ti];
     const uint32_t local = item - item_begin[ti];
@@ -322,6 +315,25 @@ __global__ __launch_bounds__(BLOCK) void walk_tiles_kernel(
         walk_band<H, true, CAPTURE, UNROLL, PIPE>(d, pass >> 1, (pass & 1u) != 0, t, active);
     else
         walk_band<H, false, CAPTURE, UNROLL, PIPE>(d, pass >> 1, (pass & 1u) != 0, t, active);
+}
+
+// One work item per block, or (PERSIST) a grid of a few blocks per CU
+// striding over all items.  With items_per_xcd != 0, consecutive block ids
+// (dispatched to the 8 XCDs round-robin) are remapped so each XCD takes a
+// contiguous run of items: one DEM region per L2.
+template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PERSIST>
+__global__ __launch_bounds__(BLOCK) void walk_tiles_kernel(
+    const TileDesc *__restrict__ tiles, const uint32_t *__restrict__ item_begin,
+    uint32_t n_tiles, uint32_t n_items, uint32_t items_per_xcd) {
+    const uint32_t b = blockIdx.x;
+    if (PERSIST) {
+        for (uint32_t item = b; item < n_items; item += gridDim.x)
+            walk_item<H, CAPTURE, BLOCK, UNROLL, PIPE>(tiles, item_begin, n_tiles, item);
+        return;
+    }
+    const uint32_t item = items_per_xcd ? (b & 7u) * items_per_xcd + (b >> 3) : b;
+    if (item >= n_items) return;
+    walk_item<H, CAPTURE, BLOCK, UNROLL, PIPE>(tiles, item_begin, n_tiles, item);
 }
 
 // Per point, in sector order: cum += forward; cum += backward (the f32
@@ -364,12 +376,27 @@ __global__ void widen_i16_kernel(const int16_t *in, float *out, size_t n) {
         out[i] = static_cast<float>(in[i]);
 }
 
-template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE>
+// Blocks of a persistent launch: enough for eight waves per SIMD.
+uint32_t persistent_blocks(uint32_t block) {
+    static uint32_t cus = 0;
+    if (!cus) {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            n <= 0)
+            n = 256;
+        cus = static_cast<uint32_t>(n);
+    }
+    return cus * (32u * 64u / block);
+}
+
+template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PERSIST>
 hipError_t launch_tiles_t(hipStream_t s, const TileDesc *tiles, const uint32_t *item_begin,
                           uint32_t n_tiles, uint32_t n_items, bool xcd_contiguous) {
-    const uint32_t per_xcd = xcd_contiguous ? (n_items + 7u) / 8u : 0u;
-    const uint32_t grid = xcd_contiguous ? per_xcd * 8u : n_items;
-    hipLaunchKernelGGL((walk_tiles_kernel<H, CAPTURE, BLOCK, UNROLL, PIPE>), dim3(grid),
+    const uint32_t per_xcd = (xcd_contiguous && !PERSIST) ? (n_items + 7u) / 8u : 0u;
+    uint32_t grid = per_xcd ? per_xcd * 8u : n_items;
+    if (PERSIST) grid = std::min(n_items, persistent_blocks(BLOCK));
+    hipLaunchKernelGGL((walk_tiles_kernel<H, CAPTURE, BLOCK, UNROLL, PIPE, PERSIST>), dim3(grid),
                        dim3(BLOCK), 0, s, tiles, item_begin, n_tiles, n_items, per_xcd);
     return hipGetLastError();
 }
@@ -379,12 +406,12 @@ hipError_t launch_tiles_h(hipStream_t s, const TileDesc *tiles, const uint32_t *
                           uint32_t n_tiles, uint32_t n_items, bool capture,
                           const WalkConfig &cfg) {
     if (capture)   // ring capture is a checking path: one shape
-        return launch_tiles_t<H, true, 256, 8, false>(s, tiles, item_begin, n_tiles, n_items,
-                                                      cfg.xcd_contiguous);
-#define TVS_SHAPE(B, U, P)                                                                \
-    if (cfg.block == B && cfg.unroll == U && cfg.pipeline == P)                           \
-        return launch_tiles_t<H, false, B, U, P>(s, tiles, item_begin, n_tiles, n_items,  \
-                                                 cfg.xcd_contiguous);
+        return launch_tiles_t<H, true, 256, 8, false, false>(s, tiles, item_begin, n_tiles,
+                                                             n_items, cfg.xcd_contiguous);
+#define TVS_SHAPE(B, U, P, S)                                                              \
+    if (cfg.block == B && cfg.unroll == U && cfg.pipeline == P && cfg.persistent == S)     \
+        return launch_tiles_t<H, false, B, U, P, S>(s, tiles, item_begin, n_tiles, n_items,\
+                                                    cfg.xcd_contiguous);
     TVS_WALK_SHAPES(TVS_SHAPE)
 #undef TVS_SHAPE
     return hipErrorInvalidValue;
@@ -406,8 +433,8 @@ hipError_t launch_walk_plain(tvs_ctx *c, uint32_t slot, uint32_t *ring_out) {
 }
 
 bool walk_shape_supported(const WalkConfig &cfg) {
-#define TVS_SHAPE(B, U, P) \
-    if (cfg.block == B && cfg.unroll == U && cfg.pipeline == P) return true;
+#define TVS_SHAPE(B, U, P, S) \
+    if (cfg.block == B && cfg.unroll == U && cfg.pipeline == P && cfg.persistent == S) return true;
     TVS_WALK_SHAPES(TVS_SHAPE)
 #undef TVS_SHAPE
     return false;

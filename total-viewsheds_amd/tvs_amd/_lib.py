@@ -99,13 +99,15 @@ class WalkShape(ctypes.Structure):
         ("unroll", ctypes.c_uint32),
         ("xcd_contiguous", ctypes.c_uint32),
         ("pipeline", ctypes.c_uint32),
+        ("persistent", ctypes.c_uint32),
         ("ms", ctypes.c_float),
     ]
 
     def as_dict(self) -> dict:
         return {"block": self.block, "unroll": self.unroll,
                 "xcd_contiguous": bool(self.xcd_contiguous),
-                "pipeline": bool(self.pipeline), "ms": self.ms}
+                "pipeline": bool(self.pipeline), "persistent": bool(self.persistent),
+                "ms": self.ms}
 
 
 class TvsError(RuntimeError):
@@ -375,9 +377,9 @@ class Engine:
         return int(v.value)
 
     def set_walk_shape(self, block: int, unroll: int, xcd_contiguous: bool = True,
-                       pipeline: bool = False) -> None:
+                       pipeline: bool = False, persistent: bool = False) -> None:
         self._c(load().tvs_set_walk_shape(self._h, ctypes.byref(
-            WalkShape(block, unroll, int(xcd_contiguous), int(pipeline), 0.0))))
+            WalkShape(block, unroll, int(xcd_contiguous), int(pipeline), int(persistent), 0.0))))
 
     def walk_shape(self) -> dict:
         w = WalkShape()
@@ -463,9 +465,9 @@ class Batch:
         self._c(load().tvs_batch_compute_bands(self._h))
 
     def set_walk_shape(self, block: int, unroll: int, xcd_contiguous: bool = False,
-                       pipeline: bool = False) -> None:
+                       pipeline: bool = False, persistent: bool = False) -> None:
         self._c(load().tvs_batch_set_walk_shape(self._h, ctypes.byref(
-            WalkShape(block, unroll, int(xcd_contiguous), int(pipeline), 0.0))))
+            WalkShape(block, unroll, int(xcd_contiguous), int(pipeline), int(persistent), 0.0))))
 
     def run(self, use_graph: bool = True) -> None:
         self._c(load().tvs_batch_run(self._h, int(use_graph)))
