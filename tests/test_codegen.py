@@ -1,7 +1,9 @@
 """Shape of the batched walk's inner loop in the gfx950 assembly (CPU only:
 hipcc cross-compiles).  Guards the properties the kernel's speed rests on:
-8 height loads with scalar offsets per chunk, the step tables read with
-scalar dwordx8 loads, and no divergent branch in the loop."""
+8 height loads with scalar offsets per chunk (typed loads that convert the
+int16 heights, so no conversion instruction), the step tables read with
+scalar dwordx8 loads, the exec-masked visibility update, and no divergent
+branch in the loop."""
 import os
 import re
 import shutil
@@ -40,19 +42,23 @@ def main_loop(body: str) -> str:
     for h in heads:
         end = next(i for i in range(h, len(lines)) if re.search(r"s_cbranch_scc\d", lines[i]))
         chunk = "\n".join(lines[h:end + 1])
-        if len(re.findall(r"buffer_load_(s|u)short", chunk)) >= 8:
+        if len(re.findall(r"buffer_load_(format_x|dword)\s", chunk)) >= 8:
             return chunk
     raise AssertionError("no unrolled loop found")
 
 
+@pytest.mark.parametrize("heights,load", [("Is", "buffer_load_format_x"),
+                                          ("If", "buffer_load_dword")])
 @pytest.mark.parametrize("pipe", ["Lb0E", "Lb1E"])
-def test_int16_walk_loop_shape(asm, pipe):
-    loop = main_loop(kernel_body(asm, "IsLb0ELi256ELi8E" + pipe))
+def test_walk_loop_shape(asm, heights, load, pipe):
+    loop = main_loop(kernel_body(asm, heights + "Lb0ELi256ELi8E" + pipe + "Lb0E"))
     assert "s_cbranch_execz" not in loop and "s_cbranch_vccz" not in loop
-    assert len(re.findall(r"buffer_load_(s|u)short\s+v\d+, v\d+, s\[\d+:\d+\], s\d+ offen", loop)) >= 8
+    assert len(re.findall(load + r"\s+v\d+, v\d+, s\[\d+:\d+\], s\d+ offen", loop)) == 8
     assert "s_load_dwordx8" in loop
+    assert "v_cvt_f32_i32" not in loop
+    assert len(re.findall(r"v_cmpx_gt_f32", loop)) == 8
     valu = len(re.findall(r"^\s+v_", loop, re.M))
-    assert valu <= 11 * 8 + 8, valu                # 11 per step (+ pipeline moves)
+    assert valu <= 9 * 8 + 8, valu                 # 9 per step (+ pipeline moves)
     assert "scratch" not in loop and "buffer_store" not in loop
 
 

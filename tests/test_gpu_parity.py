@@ -87,6 +87,32 @@ def test_small_tvs_bit_exact(small, method, dtype):
         assert np.max(np.abs(got64 - cum) / np.maximum(got64, 1e-30)) < 1e-5
 
 
+def test_full_int16_range_bit_exact(oracle):
+    """Heights over the whole int16 range (negative, -32768, 32767): the
+    batched walk's typed int16 loads give float(h) exactly, so int16 and f32
+    uploads and the oracle agree bit for bit."""
+    w, spacing, los = 101, 100.0, 3300
+    rng = np.random.default_rng(11)
+    elev = rng.integers(-32768, 32768, size=(w, w), dtype=np.int16)
+    elev[::7, ::5] = 32767
+    elev[3::11, 2::3] = -32768
+    elev[50, :] = rng.integers(-3, 4, size=w, dtype=np.int16)
+    elev = elev.reshape(-1)
+    note("int16 extremes 101x101: oracle band data + walk")
+    dem = oracle.dem_new(w, spacing, los)
+    deltas, dists = oracle.band_data_all(dem, range(180))
+    cum, _ = oracle.walk_sectors(dem, elev.astype(np.float32), 1.65, deltas, dists,
+                                 n_threads=THREADS)
+    for dtype in (np.int16, np.float32):
+        note(f"int16 extremes: batched, {np.dtype(dtype).name} upload")
+        with tvs_amd.Engine(w, spacing, los, 5.0, 1.65, 0) as e:
+            e.upload(elev.astype(dtype))
+            e.compute_bands(0, 180)
+            e.run(0, 180, "batched")
+            np.testing.assert_array_equal(e.surfaces().view(np.uint32), cum.view(np.uint32),
+                                          err_msg=np.dtype(dtype).name)
+
+
 def test_small_tvs_from_host_band_data(small):
     """tvs_set_band (band data from outside) feeds the same walk."""
     wl, dem, deltas, dists, cum = small

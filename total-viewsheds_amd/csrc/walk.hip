@@ -25,6 +25,13 @@
 #include "engine.hpp"
 #include "fastdiv.hpp"
 
+// Typed buffer load (MUBUF buffer_load_format_x), declared by its LLVM
+// intrinsic name: the texture unit converts the element with the
+// resource's data/number format.
+__device__ float tvs_raw_buffer_load_format_f32(__amdgpu_buffer_rsrc_t rsrc, int voffset,
+                                                int soffset, int aux)
+    __asm("llvm.amdgcn.raw.ptr.buffer.load.format.f32");
+
 namespace tvs {
 namespace {
 
@@ -99,13 +106,14 @@ __device__ __forceinline__ float load_height(__amdgpu_buffer_rsrc_t rsrc,
                                              uint32_t pov_bytes,
                                              uint32_t soff_bytes);
 
+// int16 heights: the resource's format (16-bit, SSCALED) makes the load
+// return float(h) exactly, so no conversion instruction is spent per step.
 template <>
 __device__ __forceinline__ float load_height<int16_t>(__amdgpu_buffer_rsrc_t rsrc,
                                                       uint32_t pov_bytes,
                                                       uint32_t soff_bytes) {
-    const int16_t h = static_cast<int16_t>(
-        __builtin_amdgcn_raw_buffer_load_b16(rsrc, pov_bytes, soff_bytes, 0));
-    return static_cast<float>(h);
+    return tvs_raw_buffer_load_format_f32(rsrc, static_cast<int>(pov_bytes),
+                                          static_cast<int>(soff_bytes), 0);
 }
 
 template <>
@@ -124,8 +132,32 @@ __device__ __forceinline__ float add_if(bool v, float surface, float w) {
     return __builtin_fmaf(v ? 1.0f : 0.0f, w, surface);
 }
 
-// Buffer resource word 3 for raw (untyped) loads on gfx9-family parts.
-constexpr int kBufferFlags = 0x00020000;
+// Buffer resource word 3 (gfx9 family): DST_SEL_X [2:0], NUM_FORMAT
+// [14:12], DATA_FORMAT [18:15].  Untyped loads need a non-zero data format
+// (32); the int16 heights use DST_SEL_X = X (4), NUM_FORMAT = SSCALED (3),
+// DATA_FORMAT = 16 (2).
+template <typename H> constexpr int kHeightRsrcFlags = 0x00020000;
+template <> constexpr int kHeightRsrcFlags<int16_t> = 4 | (3 << 12) | (2 << 15);
+
+// One step's update where the walk records no rings (kernel.rs:216, 233):
+// lanes with angle > max_angle add w to the surface (exec narrowed by
+// v_cmpx, restored at once), then max_angle = max(angle, max_angle) on all
+// lanes.  v_max_f32 equals `angle > max_angle ? angle : max_angle` for the
+// comparisons that follow: max_angle is never NaN, a NaN angle leaves it
+// (quiet NaNs only, from arithmetic), and equal values differ at most in
+// the sign of zero, which no comparison sees.
+__device__ __forceinline__ void step_update(float angle, float w, float &max_angle,
+                                            float &surface) {
+    uint64_t saved_exec;
+    asm("s_mov_b64 %[e], exec\n\t"
+        "v_cmpx_gt_f32 vcc, %[a], %[m]\n\t"
+        "v_add_f32 %[s], %[w], %[s]\n\t"
+        "s_mov_b64 exec, %[e]\n\t"
+        "v_max_f32 %[m], %[a], %[m]"
+        : [m] "+v"(max_angle), [s] "+v"(surface), [e] "=&s"(saved_exec)
+        : [a] "v"(angle), [w] "s"(w)
+        : "vcc");
+}
 
 // Pointers read out of a TileDesc are generic; naming them global lets the
 // wave-uniform table reads become scalar loads (s_load_dwordx8) instead of
@@ -156,7 +188,7 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
     const uint32_t bias = global_ptr(d.meta)[slot].bias;
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<H *>(elev - bias), 0,
-        static_cast<int>((bias + d.w * d.w) * sizeof(H)), kBufferFlags);
+        static_cast<int>((bias + d.w * d.w) * sizeof(H)), kHeightRsrcFlags<H>);
 
     const size_t tb = static_cast<size_t>(slot) * d.stride;
     // Element position of each step relative to the resource base:
@@ -214,8 +246,12 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
             const float q = FAST ? div_by_reciprocal(dh, dist[k0 + u], rcp[k0 + u], rcp_lo[k0 + u])
                                  : dh / dist[k0 + u];
             const float angle = q - curv[k0 + u];
+            if (!CAPTURE) {
+                step_update(angle, wt[k0 + u], max_angle, surface);
+                continue;
+            }
             const bool v = angle > max_angle;
-            if (CAPTURE && capture && v != visible) event(pos[k0 + u]);
+            if (capture && v != visible) event(pos[k0 + u]);
             surface = add_if(v, surface, wt[k0 + u]);
             max_angle = v ? angle : max_angle;
             visible = v;
