@@ -214,10 +214,13 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true", help="skip the plain method")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default walk shape")
+    ap.add_argument("--shape", metavar="B,U,X,P,S",
+                    help="walk shape instead of tuning: points per item, unroll, "
+                         "XCD-contiguous, pipeline, persistent (e.g. 64,8,1,0,0)")
     ap.add_argument("--no-f32", action="store_true", help="skip the f32-heights batched run")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the large-3601 and tiles-64 figures")
-    ap.add_argument("--cpu-sectors", type=int, default=18)
+    ap.add_argument("--cpu-sectors", type=int, default=36)
     ap.add_argument("--cpu-threads", type=int, default=1)
     ap.add_argument("--copy-calibration", type=int, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration)")
@@ -253,8 +256,11 @@ def main() -> int:
     point_steps = 180 * 2 * g.computable_points_count * g.band_deltas_size
     real_point_steps = 180 * 2 * g.computable_points_count * g.real_steps
 
+    if args.shape:
+        b, u, x, p, ps = (int(v) for v in args.shape.split(","))
+        e.set_walk_shape(b, u, bool(x), bool(p), bool(ps))
     shape = e.walk_shape()
-    if not args.no_tune:
+    if not args.no_tune and not args.shape:
         shape, table = e.autotune(0, 180, reps=1)
         log(f"autotuned walk shape: block {shape['block']}, unroll {shape['unroll']}, "
             f"xcd-contiguous {shape['xcd_contiguous']}, pipeline {shape['pipeline']}, "
@@ -372,6 +378,13 @@ def main() -> int:
             f"{args.cpu_threads} thread(s)")
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_sectors, args.cpu_threads)
         log(f"CPU: {out['cpu_baseline']['value']:.3e} point-steps/s")
+        # The same sample over every host core this process may use (capped
+        # at 16, the host share of one GPU), reported beside the contract's
+        # single-thread figure.
+        cores = min(16, len(os.sched_getaffinity(0)))
+        if cores > 1:
+            out["cpu_all_cores"] = cpu_baseline(wl, args.cpu_sectors, cores)
+            log(f"CPU, {cores} threads: {out['cpu_all_cores']['value']:.3e} point-steps/s")
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
