@@ -244,9 +244,11 @@ const char *tvs_status_string(tvs_status status);
 
 /* Points per work item (threads per block), steps unrolled per chunk,
  * whether consecutive work items stay on one XCD, and whether the next
- * chunk's heights are requested before the current chunk's arithmetic, and
+ * chunk's heights are requested before the current chunk's arithmetic,
  * whether a grid of a few blocks per CU strides over all work items
- * (`persistent`, which ignores `xcd_contiguous`); `ms` is a timing result.
+ * (`persistent`, which ignores `xcd_contiguous`), and whether one lane walks
+ * both bands (forward and backward) of its point, sharing the per-step
+ * constants (`both_directions`); `ms` is a timing result.
  * No reference counterpart: the Vulkan path's fixed 8x8x4 workgroup
  * (lib.rs:23, gpu.rs:10) is the only shape the reference has. */
 typedef struct tvs_walk_shape {
@@ -255,6 +257,7 @@ typedef struct tvs_walk_shape {
     uint32_t xcd_contiguous;
     uint32_t pipeline;
     uint32_t persistent;
+    uint32_t both_directions;
     float ms;
 } tvs_walk_shape;
 

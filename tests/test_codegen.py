@@ -51,7 +51,7 @@ def main_loop(body: str) -> str:
                                           ("If", "buffer_load_dword")])
 @pytest.mark.parametrize("pipe", ["Lb0E", "Lb1E"])
 def test_walk_loop_shape(asm, heights, load, pipe):
-    loop = main_loop(kernel_body(asm, heights + "Lb0ELi256ELi8E" + pipe + "Lb0E"))
+    loop = main_loop(kernel_body(asm, heights + "Lb0ELi256ELi8E" + pipe + "Lb0ELb0E"))
     assert "s_cbranch_execz" not in loop and "s_cbranch_vccz" not in loop
     assert len(re.findall(load + r"\s+v\d+, v\d+, s\[\d+:\d+\], s\d+ offen", loop)) == 8
     assert "s_load_dwordx8" in loop
@@ -59,6 +59,19 @@ def test_walk_loop_shape(asm, heights, load, pipe):
     assert len(re.findall(r"v_cmpx_gt_f32", loop)) == 8
     valu = len(re.findall(r"^\s+v_", loop, re.M))
     assert valu <= 9 * 8 + 8, valu                 # 9 per step (+ pipeline moves)
+    assert "scratch" not in loop and "buffer_store" not in loop
+
+
+def test_paired_walk_loop_shape(asm):
+    """Both bands of a point per lane: 16 height loads and 16 masked updates
+    per 8-step chunk, still 9 VALU per band-step, one set of table reads."""
+    loop = main_loop(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb1E"))
+    assert "s_cbranch_execz" not in loop and "s_cbranch_vccz" not in loop
+    assert len(re.findall(r"buffer_load_format_x\s+v\d+, v\d+, s\[\d+:\d+\], s\d+ offen", loop)) == 16
+    assert len(re.findall(r"v_cmpx_gt_f32", loop)) == 16
+    assert len(re.findall(r"s_load_dwordx8", loop)) == 7     # pos_f, pos_b + 5 shared tables
+    valu = len(re.findall(r"^\s+v_", loop, re.M))
+    assert valu <= 9 * 16 + 8, valu
     assert "scratch" not in loop and "buffer_store" not in loop
 
 

@@ -87,7 +87,7 @@ def test_autotune_keeps_results_exact(tiles_and_truth):
         before = e.surfaces()
         best, table = e.autotune(0, 30, reps=1)
         print("autotune", best, flush=True)
-        assert len(table) == 2 * 11 + 3 and best["ms"] > 0   # persistent shapes once
+        assert len(table) == 2 * 16 + 3 and best["ms"] > 0   # persistent shapes once
         np.testing.assert_array_equal(e.surfaces(), before)   # surfaces preserved
         assert e.walk_shape()["block"] == best["block"]
         e.reset_surfaces()

@@ -298,12 +298,14 @@ def test_ring_overflow_is_counted_not_spilled(small):
         assert np.all(r[:, 0] % 2 == 0)
 
 
-SHAPES = [  # (block, unroll, pipeline, persistent)
-    (256, 8, False, False), (128, 8, False, False), (64, 8, False, False),
-    (512, 8, False, False), (256, 4, False, False), (256, 16, False, False),
-    (128, 16, False, False), (256, 8, True, False), (128, 8, True, False),
-    (256, 4, True, False), (512, 8, True, False), (256, 8, False, True),
-    (128, 8, False, True), (64, 8, False, True)]
+SHAPES = [  # (block, unroll, pipeline, persistent, both directions per lane)
+    (256, 8, False, False, False), (128, 8, False, False, False), (64, 8, False, False, False),
+    (512, 8, False, False, False), (256, 4, False, False, False), (256, 16, False, False, False),
+    (128, 16, False, False, False), (256, 8, True, False, False), (128, 8, True, False, False),
+    (256, 4, True, False, False), (512, 8, True, False, False), (256, 8, False, True, False),
+    (128, 8, False, True, False), (64, 8, False, True, False), (64, 8, False, False, True),
+    (128, 8, False, False, True), (256, 8, False, False, True), (64, 4, False, False, True),
+    (128, 4, False, False, True)]
 
 
 @pytest.mark.parametrize("xcd", [True, False])
@@ -314,11 +316,11 @@ def test_every_walk_shape_bit_exact(small, xcd):
     with make_engine(wl) as e:
         e.upload(wl.elevations)
         e.compute_bands(0, 180)
-        for block, unroll, pipe, pers in SHAPES:
-            note(f"shape {block}x{unroll} pipe={pipe} pers={pers} xcd={xcd}")
-            e.set_walk_shape(block, unroll, xcd, pipe, pers)
+        for block, unroll, pipe, pers, pair in SHAPES:
+            note(f"shape {block}x{unroll} pipe={pipe} pers={pers} pair={pair} xcd={xcd}")
+            e.set_walk_shape(block, unroll, xcd, pipe, pers, pair)
             e.reset_surfaces()
             e.run(0, 180, "batched")
             np.testing.assert_array_equal(e.surfaces(), cum,
                                           err_msg=f"shape {block}x{unroll} pipe={pipe} "
-                                                  f"pers={pers}")
+                                                  f"pers={pers} pair={pair}")

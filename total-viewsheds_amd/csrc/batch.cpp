@@ -249,7 +249,8 @@ tvs_status tvs_batch_compute_bands(tvs_batch *b) {
 tvs_status tvs_batch_set_walk_shape(tvs_batch *b, const tvs_walk_shape *shape) {
     if (!b || !shape) return bfail(b, TVS_ERR_INVALID_ARGUMENT, "null argument");
     tvs::WalkConfig cfg{shape->block, shape->unroll, shape->xcd_contiguous != 0,
-                        shape->pipeline != 0, shape->persistent != 0};
+                        shape->pipeline != 0, shape->persistent != 0,
+                        shape->both_directions != 0};
     if (!tvs::walk_shape_supported(cfg))
         return bfail(b, TVS_ERR_INVALID_ARGUMENT, "walk shape not compiled");
     b->cfg = cfg;

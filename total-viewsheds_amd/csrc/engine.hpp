@@ -102,15 +102,21 @@ struct WalkConfig {
     bool xcd_contiguous = true;
     bool pipeline = false;    // next chunk's heights requested before this chunk's math
     bool persistent = false;  // a few blocks per CU stride over all items
+    bool pair = false;        // one lane walks both bands (forward, backward) of a point
 };
 
-// Compiled (block, unroll, pipeline, persistent) shapes of the tile walk.
+// Compiled (block, unroll, pipeline, persistent, pair) shapes of the tile walk.
 #define TVS_WALK_SHAPES(X)                                                               \
-    X(256, 8, false, false) X(128, 8, false, false) X(64, 8, false, false)               \
-    X(512, 8, false, false) X(256, 4, false, false) X(256, 16, false, false)             \
-    X(128, 16, false, false) X(256, 8, true, false) X(128, 8, true, false)               \
-    X(256, 4, true, false) X(512, 8, true, false) X(256, 8, false, true)                 \
-    X(128, 8, false, true) X(64, 8, false, true)
+    X(256, 8, false, false, false) X(128, 8, false, false, false)                        \
+    X(64, 8, false, false, false) X(512, 8, false, false, false)                         \
+    X(256, 4, false, false, false) X(256, 16, false, false, false)                       \
+    X(128, 16, false, false, false) X(256, 8, true, false, false)                        \
+    X(128, 8, true, false, false) X(256, 4, true, false, false)                          \
+    X(512, 8, true, false, false) X(256, 8, false, true, false)                          \
+    X(128, 8, false, true, false) X(64, 8, false, true, false)                           \
+    X(64, 8, false, false, true) X(128, 8, false, false, true)                           \
+    X(256, 8, false, false, true) X(64, 4, false, false, true)                           \
+    X(128, 4, false, false, true)
 
 // Scratch for the device Axes::compute (keys, ids, radix-sort ping-pong,
 // histograms).

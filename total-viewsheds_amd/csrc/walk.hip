@@ -159,6 +159,27 @@ __device__ __forceinline__ void step_update(float angle, float w, float &max_ang
         : "vcc");
 }
 
+// step_update for a lane's forward (f) and backward (b) band at the same
+// step: one exec save serves both masked adds.
+__device__ __forceinline__ void pair_update(float angle_f, float angle_b, float w,
+                                            float &max_f, float &max_b, float &surface_f,
+                                            float &surface_b) {
+    uint64_t saved_exec;
+    asm("s_mov_b64 %[e], exec\n\t"
+        "v_cmpx_gt_f32 vcc, %[af], %[mf]\n\t"
+        "v_add_f32 %[sf], %[w], %[sf]\n\t"
+        "s_mov_b64 exec, %[e]\n\t"
+        "v_cmpx_gt_f32 vcc, %[ab], %[mb]\n\t"
+        "v_add_f32 %[sb], %[w], %[sb]\n\t"
+        "s_mov_b64 exec, %[e]\n\t"
+        "v_max_f32 %[mf], %[af], %[mf]\n\t"
+        "v_max_f32 %[mb], %[ab], %[mb]"
+        : [mf] "+v"(max_f), [mb] "+v"(max_b), [sf] "+v"(surface_f), [sb] "+v"(surface_b),
+          [e] "=&s"(saved_exec)
+        : [af] "v"(angle_f), [ab] "v"(angle_b), [w] "s"(w)
+        : "vcc");
+}
+
 // Pointers read out of a TileDesc are generic; naming them global lets the
 // wave-uniform table reads become scalar loads (s_load_dwordx8) instead of
 // per-lane loads plus a readfirstlane loop for the scalar offset.
@@ -324,6 +345,77 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
     if (active) global_ptr(d.partial)[static_cast<size_t>(pass) * d.N + t] = surface;
 }
 
+// Both bands of point t in sector slot d.first_slot + rel in one lane, no
+// ring capture.  The forward and backward walks share every per-step
+// constant (distance, reciprocal, curvature, weight) and differ only in the
+// step's position, so each scalar table read and each loop iteration's
+// bookkeeping serves two bands, and the lane carries two independent
+// max-scan chains.  The zero tail is skipped: its angle is the same at every
+// step and its weight is +0, so it changes neither surface.
+template <typename H, bool FAST, int UNROLL>
+__device__ __forceinline__ void walk_pair(const TileDesc &d, uint32_t rel, uint32_t t,
+                                          bool active) {
+    const H *elev = static_cast<const H *>(d.elev);
+    const uint32_t slot = d.first_slot + rel;
+    const uint32_t pov = (t / d.T + d.L) * d.w + (t % d.T + d.L);
+    const uint32_t pov_bytes = pov * static_cast<uint32_t>(sizeof(H));
+    const float h0 = static_cast<float>(global_ptr(elev)[pov]) + d.obs;
+    const uint32_t n_fast = global_ptr(d.meta)[slot].n_fast;
+    const uint32_t tail_start = global_ptr(d.meta)[slot].tail_start;
+    const uint32_t bias = global_ptr(d.meta)[slot].bias;
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<H *>(elev - bias), 0,
+        static_cast<int>((bias + d.w * d.w) * sizeof(H)), kHeightRsrcFlags<H>);
+
+    const size_t tb = static_cast<size_t>(slot) * d.stride;
+    const TVS_GLOBAL uint32_t *__restrict__ pos_f = global_ptr(d.pos_f + tb);
+    const TVS_GLOBAL uint32_t *__restrict__ pos_b = global_ptr(d.pos_b + tb);
+    auto soff = [&](uint32_t p) -> uint32_t { return p * static_cast<uint32_t>(sizeof(H)); };
+    const TVS_GLOBAL float *__restrict__ dist = global_ptr(d.dist + tb);
+    const TVS_GLOBAL float *__restrict__ rcp = global_ptr(d.rcp + tb);
+    const TVS_GLOBAL float *__restrict__ rcp_lo = global_ptr(d.rcp_lo + tb);
+    const TVS_GLOBAL float *__restrict__ curv = global_ptr(d.curv + tb);
+    const TVS_GLOBAL float *__restrict__ wt = global_ptr(d.wt + tb);
+
+    float max_f = kMaxAngle, max_b = kMaxAngle;
+    float surface_f = 0.0f, surface_b = 0.0f;
+    uint32_t k = 0;
+    const uint32_t fast_end = n_fast - n_fast % UNROLL;
+    for (; k < fast_end; k += UNROLL) {
+        float hf[UNROLL], hb[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            hf[u] = load_height<H>(rsrc, pov_bytes, soff(pos_f[k + u]));
+            hb[u] = load_height<H>(rsrc, pov_bytes, soff(pos_b[k + u]));
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            const float dhf = hf[u] - h0;
+            const float dhb = hb[u] - h0;
+            const float qf = FAST ? div_by_reciprocal(dhf, dist[k + u], rcp[k + u], rcp_lo[k + u])
+                                  : dhf / dist[k + u];
+            const float qb = FAST ? div_by_reciprocal(dhb, dist[k + u], rcp[k + u], rcp_lo[k + u])
+                                  : dhb / dist[k + u];
+            pair_update(qf - curv[k + u], qb - curv[k + u], wt[k + u], max_f, max_b, surface_f,
+                        surface_b);
+        }
+    }
+    for (; k < tail_start; ++k) {
+        const float dk = dist[k];
+        const float dhf = load_height<H>(rsrc, pov_bytes, soff(pos_f[k])) - h0;
+        const float dhb = load_height<H>(rsrc, pov_bytes, soff(pos_b[k])) - h0;
+        const bool fast_k = FAST && k < n_fast;
+        const float qf = fast_k ? div_by_reciprocal(dhf, dk, rcp[k], rcp_lo[k]) : dhf / dk;
+        const float qb = fast_k ? div_by_reciprocal(dhb, dk, rcp[k], rcp_lo[k]) : dhb / dk;
+        pair_update(qf - curv[k], qb - curv[k], wt[k], max_f, max_b, surface_f, surface_b);
+    }
+    if (active) {
+        TVS_GLOBAL float *out = global_ptr(d.partial) + static_cast<size_t>(2u * rel) * d.N + t;
+        out[0] = surface_f;
+        out[d.N] = surface_b;
+    }
+}
+
 // Index of the tile whose [begin[i], begin[i+1]) holds x (wave-uniform).
 __device__ __forceinline__ uint32_t find_tile(const uint32_t *__restrict__ begin,
                                               uint32_t n_tiles, uint32_t x) {
@@ -335,10 +427,30 @@ __device__ __forceinline__ uint32_t find_tile(const uint32_t *__restrict__ begin
     return lo;
 }
 
-template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE>
+// A work item is (tile, sector, direction, BLOCK points).  With PAIR an item
+// is (tile, sector, BLOCK points), both directions: `item` counts pairs and
+// 2*item is the first of the pair's two single-direction items (every tile
+// holds an even number of them).
+template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PAIR>
 __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
                                           const uint32_t *__restrict__ item_begin,
                                           uint32_t n_tiles, uint32_t item) {
+    if (PAIR) {
+        const uint32_t x = 2u * item;
+        const uint32_t ti = n_tiles > 1 ? find_tile(item_begin, n_tiles, x) : 0u;
+        const TileDesc d = tiles[ti];
+        const uint32_t local = (x - item_begin[ti]) >> 1;
+        const uint32_t rel = local / d.blocks_per_pass;
+        const uint32_t ob = local - rel * d.blocks_per_pass;
+        const uint32_t t_raw = ob * BLOCK + threadIdx.x;
+        const bool active = t_raw < d.N;
+        const uint32_t t = active ? t_raw : d.N - 1u;
+        if (d.fast)
+            walk_pair<H, true, UNROLL>(d, rel, t, active);
+        else
+            walk_pair<H, false, UNROLL>(d, rel, t, active);
+        return;
+    }
     const uint32_t ti = n_tiles > 1 ? find_tile(item_begin, n_tiles, item) : 0u;
     const TileDesc d = tiles[ti];
     const uint32_t local = item - item_begin[ti];
@@ -357,19 +469,19 @@ __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
 // striding over all items.  With items_per_xcd != 0, consecutive block ids
 // (dispatched to the 8 XCDs round-robin) are remapped so each XCD takes a
 // contiguous run of items: one DEM region per L2.
-template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PERSIST>
+template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PERSIST, bool PAIR>
 __global__ __launch_bounds__(BLOCK) void walk_tiles_kernel(
     const TileDesc *__restrict__ tiles, const uint32_t *__restrict__ item_begin,
     uint32_t n_tiles, uint32_t n_items, uint32_t items_per_xcd) {
     const uint32_t b = blockIdx.x;
     if (PERSIST) {
         for (uint32_t item = b; item < n_items; item += gridDim.x)
-            walk_item<H, CAPTURE, BLOCK, UNROLL, PIPE>(tiles, item_begin, n_tiles, item);
+            walk_item<H, CAPTURE, BLOCK, UNROLL, PIPE, PAIR>(tiles, item_begin, n_tiles, item);
         return;
     }
     const uint32_t item = items_per_xcd ? (b & 7u) * items_per_xcd + (b >> 3) : b;
     if (item >= n_items) return;
-    walk_item<H, CAPTURE, BLOCK, UNROLL, PIPE>(tiles, item_begin, n_tiles, item);
+    walk_item<H, CAPTURE, BLOCK, UNROLL, PIPE, PAIR>(tiles, item_begin, n_tiles, item);
 }
 
 // Per point, in sector order: cum += forward; cum += backward (the f32
@@ -426,14 +538,18 @@ uint32_t persistent_blocks(uint32_t block) {
     return cus * (32u * 64u / block);
 }
 
-template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PERSIST>
+// `n_items` counts single-direction items; a PAIR launch has half as many
+// (every tile's count is even).
+template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PERSIST, bool PAIR>
 hipError_t launch_tiles_t(hipStream_t s, const TileDesc *tiles, const uint32_t *item_begin,
                           uint32_t n_tiles, uint32_t n_items, bool xcd_contiguous) {
-    const uint32_t per_xcd = (xcd_contiguous && !PERSIST) ? (n_items + 7u) / 8u : 0u;
-    uint32_t grid = per_xcd ? per_xcd * 8u : n_items;
-    if (PERSIST) grid = std::min(n_items, persistent_blocks(BLOCK));
-    hipLaunchKernelGGL((walk_tiles_kernel<H, CAPTURE, BLOCK, UNROLL, PIPE, PERSIST>), dim3(grid),
-                       dim3(BLOCK), 0, s, tiles, item_begin, n_tiles, n_items, per_xcd);
+    if (PAIR && (n_items & 1u)) return hipErrorInvalidValue;
+    const uint32_t items = PAIR ? n_items / 2u : n_items;
+    const uint32_t per_xcd = (xcd_contiguous && !PERSIST) ? (items + 7u) / 8u : 0u;
+    uint32_t grid = per_xcd ? per_xcd * 8u : items;
+    if (PERSIST) grid = std::min(items, persistent_blocks(BLOCK));
+    hipLaunchKernelGGL((walk_tiles_kernel<H, CAPTURE, BLOCK, UNROLL, PIPE, PERSIST, PAIR>),
+                       dim3(grid), dim3(BLOCK), 0, s, tiles, item_begin, n_tiles, items, per_xcd);
     return hipGetLastError();
 }
 
@@ -442,12 +558,13 @@ hipError_t launch_tiles_h(hipStream_t s, const TileDesc *tiles, const uint32_t *
                           uint32_t n_tiles, uint32_t n_items, bool capture,
                           const WalkConfig &cfg) {
     if (capture)   // ring capture is a checking path: one shape
-        return launch_tiles_t<H, true, 256, 8, false, false>(s, tiles, item_begin, n_tiles,
-                                                             n_items, cfg.xcd_contiguous);
-#define TVS_SHAPE(B, U, P, S)                                                              \
-    if (cfg.block == B && cfg.unroll == U && cfg.pipeline == P && cfg.persistent == S)     \
-        return launch_tiles_t<H, false, B, U, P, S>(s, tiles, item_begin, n_tiles, n_items,\
-                                                    cfg.xcd_contiguous);
+        return launch_tiles_t<H, true, 256, 8, false, false, false>(
+            s, tiles, item_begin, n_tiles, n_items, cfg.xcd_contiguous);
+#define TVS_SHAPE(B, U, P, S, D)                                                           \
+    if (cfg.block == B && cfg.unroll == U && cfg.pipeline == P && cfg.persistent == S &&   \
+        cfg.pair == D)                                                                     \
+        return launch_tiles_t<H, false, B, U, P, S, D>(s, tiles, item_begin, n_tiles,      \
+                                                       n_items, cfg.xcd_contiguous);
     TVS_WALK_SHAPES(TVS_SHAPE)
 #undef TVS_SHAPE
     return hipErrorInvalidValue;
@@ -469,8 +586,10 @@ hipError_t launch_walk_plain(tvs_ctx *c, uint32_t slot, uint32_t *ring_out) {
 }
 
 bool walk_shape_supported(const WalkConfig &cfg) {
-#define TVS_SHAPE(B, U, P, S) \
-    if (cfg.block == B && cfg.unroll == U && cfg.pipeline == P && cfg.persistent == S) return true;
+#define TVS_SHAPE(B, U, P, S, D)                                                         \
+    if (cfg.block == B && cfg.unroll == U && cfg.pipeline == P && cfg.persistent == S &&   \
+        cfg.pair == D)                                                                     \
+        return true;
     TVS_WALK_SHAPES(TVS_SHAPE)
 #undef TVS_SHAPE
     return false;

@@ -100,6 +100,7 @@ class WalkShape(ctypes.Structure):
         ("xcd_contiguous", ctypes.c_uint32),
         ("pipeline", ctypes.c_uint32),
         ("persistent", ctypes.c_uint32),
+        ("both_directions", ctypes.c_uint32),
         ("ms", ctypes.c_float),
     ]
 
@@ -107,7 +108,7 @@ class WalkShape(ctypes.Structure):
         return {"block": self.block, "unroll": self.unroll,
                 "xcd_contiguous": bool(self.xcd_contiguous),
                 "pipeline": bool(self.pipeline), "persistent": bool(self.persistent),
-                "ms": self.ms}
+                "both_directions": bool(self.both_directions), "ms": self.ms}
 
 
 class TvsError(RuntimeError):
@@ -377,9 +378,11 @@ class Engine:
         return int(v.value)
 
     def set_walk_shape(self, block: int, unroll: int, xcd_contiguous: bool = True,
-                       pipeline: bool = False, persistent: bool = False) -> None:
+                       pipeline: bool = False, persistent: bool = False,
+                       both_directions: bool = False) -> None:
         self._c(load().tvs_set_walk_shape(self._h, ctypes.byref(
-            WalkShape(block, unroll, int(xcd_contiguous), int(pipeline), int(persistent), 0.0))))
+            WalkShape(block, unroll, int(xcd_contiguous), int(pipeline), int(persistent),
+                      int(both_directions), 0.0))))
 
     def walk_shape(self) -> dict:
         w = WalkShape()
@@ -465,9 +468,11 @@ class Batch:
         self._c(load().tvs_batch_compute_bands(self._h))
 
     def set_walk_shape(self, block: int, unroll: int, xcd_contiguous: bool = False,
-                       pipeline: bool = False, persistent: bool = False) -> None:
+                       pipeline: bool = False, persistent: bool = False,
+                       both_directions: bool = False) -> None:
         self._c(load().tvs_batch_set_walk_shape(self._h, ctypes.byref(
-            WalkShape(block, unroll, int(xcd_contiguous), int(pipeline), int(persistent), 0.0))))
+            WalkShape(block, unroll, int(xcd_contiguous), int(pipeline), int(persistent),
+                      int(both_directions), 0.0))))
 
     def run(self, use_graph: bool = True) -> None:
         self._c(load().tvs_batch_run(self._h, int(use_graph)))
