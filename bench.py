@@ -120,6 +120,9 @@ def measure_tiles(steps: int, warmup: int) -> dict:
             b.upload(i, t.elevations)
         b.compute_bands()
         bands_ms = b.timing().bands_ms
+        shape, _ = b.autotune(reps=1)
+        log(f"tiles-64 tuned shape: block {shape['block']}, unroll {shape['unroll']}, "
+            f"xcd {shape['xcd_contiguous']}, both directions {shape['both_directions']}")
         for _ in range(warmup):
             b.run(True)
         b.sync()
@@ -139,7 +142,9 @@ def measure_tiles(steps: int, warmup: int) -> dict:
             "kernel_ms": tm.walk_ms, "reduce_ms": tm.reduce_ms,
             "point_steps_per_step": ps, "bands_ms": bands_ms,
             "tiles": len(tiles),
-            "sides": sorted({t.width for t in tiles})}
+            "sides": sorted({t.width for t in tiles}),
+            "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous",
+                                                 "pipeline", "persistent", "both_directions")}}
 
 
 def copy_calibration(mib: int) -> int:

@@ -297,6 +297,11 @@ tvs_status tvs_batch_upload_f32(tvs_batch *batch, uint32_t tile,
 /* Band data for all 180 sectors of every distinct (width, band size). */
 tvs_status tvs_batch_compute_bands(tvs_batch *batch);
 tvs_status tvs_batch_set_walk_shape(tvs_batch *batch, const tvs_walk_shape *shape);
+/* tvs_autotune for a batch: times the walk launch over all tiles in every
+ * compiled shape and keeps the fastest.  Only the walk runs, so the tiles'
+ * cumulative surfaces are unchanged.  Needs band data and elevations. */
+tvs_status tvs_batch_autotune(tvs_batch *batch, uint32_t reps, tvs_walk_shape *results,
+                              size_t max_results, size_t *n_results, tvs_walk_shape *best);
 /* All tiles x 180 sectors; `use_graph` replays the captured launch pair.
  * Asynchronous; surfaces accumulate as in tvs_run_sectors. */
 tvs_status tvs_batch_run(tvs_batch *batch, uint32_t use_graph);

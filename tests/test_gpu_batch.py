@@ -66,10 +66,14 @@ def test_batch_shapes_agree(tiles_and_truth):
         for i, t in enumerate(tiles):
             b.upload(i, t.elevations)
         b.compute_bands()
-        for block, unroll, xcd, pipe in [(64, 8, False, False), (128, 16, True, False),
-                                         (512, 8, False, True), (256, 4, True, True)]:
-            print(f"batch shape {block}x{unroll} xcd={xcd} pipe={pipe}", flush=True)
-            b.set_walk_shape(block, unroll, xcd, pipe)
+        for block, unroll, xcd, pipe, pair in [(64, 8, False, False, False),
+                                               (128, 16, True, False, False),
+                                               (512, 8, False, True, False),
+                                               (256, 4, True, True, False),
+                                               (64, 8, False, False, True),
+                                               (128, 4, True, False, True)]:
+            print(f"batch shape {block}x{unroll} xcd={xcd} pipe={pipe} pair={pair}", flush=True)
+            b.set_walk_shape(block, unroll, xcd, pipe, both_directions=pair)
             b.reset_surfaces()
             b.run(True)
             for i in range(len(tiles)):
@@ -93,3 +97,25 @@ def test_autotune_keeps_results_exact(tiles_and_truth):
         e.reset_surfaces()
         e.run(0, 180, "batched")
         np.testing.assert_array_equal(e.surfaces(), truth[0])
+
+
+def test_batch_autotune_keeps_results_exact(tiles_and_truth):
+    """tvs_batch_autotune times only the walk: the cumulative surfaces are
+    unchanged, and the picked shape's run is bit-exact."""
+    tiles, truth = tiles_and_truth
+    with tvs_amd.Batch([(t.width, t.spacing, t.max_line_of_sight, t.rings_per_km,
+                         t.observer_height) for t in tiles]) as b:
+        for i, t in enumerate(tiles):
+            b.upload(i, t.elevations)
+        b.compute_bands()
+        b.run(True)
+        before = [b.surfaces(i) for i in range(len(tiles))]
+        best, table = b.autotune(reps=1)
+        print("batch autotune", best, flush=True)
+        assert len(table) == 2 * 16 + 3 and best["ms"] > 0
+        for i in range(len(tiles)):
+            np.testing.assert_array_equal(b.surfaces(i), before[i])
+        b.reset_surfaces()
+        b.run(True)
+        for i in range(len(tiles)):
+            np.testing.assert_array_equal(b.surfaces(i), truth[i], err_msg=f"tile {i}")

@@ -258,6 +258,62 @@ tvs_status tvs_batch_set_walk_shape(tvs_batch *b, const tvs_walk_shape *shape) {
     return TVS_OK;
 }
 
+tvs_status tvs_batch_autotune(tvs_batch *b, uint32_t reps, tvs_walk_shape *results,
+                              size_t max_results, size_t *n_results, tvs_walk_shape *best) {
+    if (!b) return bfail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null batch");
+    if (reps == 0) reps = 1;
+    // Only the walk launch is timed: it writes the per-sector band surfaces
+    // (scratch), never the cumulative surfaces, so the caller's totals stay.
+    TVS_BHIP(b, hipStreamSynchronize(b->stream), "sync");
+    const tvs::WalkConfig keep = b->cfg;
+    tvs::WalkConfig winner = keep;
+    float winner_ms = 1e30f;
+    size_t k = 0;
+    tvs_status st = TVS_OK;
+    for (const tvs::WalkConfig &cfg : tvs::compiled_walk_shapes()) {
+        b->cfg = cfg;
+        drop_graph(b);
+        st = prepare(b);
+        if (st != TVS_OK) break;
+        auto launch = [&]() {
+            return tvs::launch_walk_tiles(b->stream, b->d_tiles, b->d_item_begin,
+                                          static_cast<uint32_t>(b->tiles.size()), b->n_items,
+                                          b->i16, false, b->cfg);
+        };
+        hipError_t e = launch();   // warm-up
+        if (e == hipSuccess) e = hipEventRecord(b->ev[0], b->stream);
+        for (uint32_t r = 0; r < reps && e == hipSuccess; ++r) e = launch();
+        if (e == hipSuccess) e = hipEventRecord(b->ev[1], b->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(b->ev[1]);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
+        if (e != hipSuccess) {
+            st = bfail(b, TVS_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+            break;
+        }
+        ms /= static_cast<float>(reps);
+        if (results && k < max_results)
+            results[k] = tvs_walk_shape{cfg.block, cfg.unroll, cfg.xcd_contiguous ? 1u : 0u,
+                                        cfg.pipeline ? 1u : 0u, cfg.persistent ? 1u : 0u,
+                                        cfg.pair ? 1u : 0u, ms};
+        ++k;
+        if (ms < winner_ms) {
+            winner_ms = ms;
+            winner = cfg;
+        }
+    }
+    b->cfg = st == TVS_OK ? winner : keep;
+    drop_graph(b);
+    b->ran = false;   // the run events now hold tuner timings
+    if (st != TVS_OK) return st;
+    if (n_results) *n_results = k;
+    if (best)
+        *best = tvs_walk_shape{winner.block, winner.unroll, winner.xcd_contiguous ? 1u : 0u,
+                               winner.pipeline ? 1u : 0u, winner.persistent ? 1u : 0u,
+                               winner.pair ? 1u : 0u, winner_ms};
+    return TVS_OK;
+}
+
 tvs_status tvs_batch_run(tvs_batch *b, uint32_t use_graph) {
     if (!b) return bfail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null batch");
     tvs_status st = prepare(b);

@@ -190,6 +190,9 @@ uint32_t band_select_capacity();
 // ---- launchers (walk.hip) ----
 hipError_t launch_walk_plain(tvs_ctx *c, uint32_t slot, uint32_t *ring_out);
 bool walk_shape_supported(const WalkConfig &cfg);
+// Every compiled shape the tuners time: both item mappings, persistent
+// shapes once.
+std::vector<WalkConfig> compiled_walk_shapes();
 hipError_t launch_walk_tiles(hipStream_t s, const TileDesc *tiles,
                              const uint32_t *item_begin, uint32_t n_tiles,
                              uint32_t n_items, bool i16, bool capture,

@@ -12,18 +12,22 @@
 
 #include "engine.hpp"
 
-namespace {
+namespace tvs {
 
-std::vector<tvs::WalkConfig> compiled_shapes() {
-    std::vector<tvs::WalkConfig> out;
+std::vector<WalkConfig> compiled_walk_shapes() {
+    std::vector<WalkConfig> out;
     // A persistent grid strides over the items itself: one item mapping.
 #define TVS_SHAPE(B, U, P, S, D)                                   \
-    if (!S) out.push_back(tvs::WalkConfig{B, U, true, P, S, D});   \
-    out.push_back(tvs::WalkConfig{B, U, false, P, S, D});
+    if (!S) out.push_back(WalkConfig{B, U, true, P, S, D});   \
+    out.push_back(WalkConfig{B, U, false, P, S, D});
     TVS_WALK_SHAPES(TVS_SHAPE)
 #undef TVS_SHAPE
     return out;
 }
+
+}  // namespace tvs
+
+namespace {
 
 tvs_status tfail(tvs_ctx *c, tvs_status st, const std::string &msg) {
     c->err = msg;
@@ -57,7 +61,7 @@ tvs_status tvs_get_walk_shape(const tvs_ctx *c, tvs_walk_shape *out) {
     return TVS_OK;
 }
 
-size_t tvs_walk_shape_count(void) { return compiled_shapes().size(); }
+size_t tvs_walk_shape_count(void) { return tvs::compiled_walk_shapes().size(); }
 
 tvs_status tvs_autotune(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t reps,
                         tvs_walk_shape *results, size_t max_results, size_t *n_results,
@@ -85,7 +89,7 @@ tvs_status tvs_autotune(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t rep
     float winner_ms = 1e30f;
     size_t k = 0;
     tvs_status st = TVS_OK;
-    for (const tvs::WalkConfig &cfg : compiled_shapes()) {
+    for (const tvs::WalkConfig &cfg : tvs::compiled_walk_shapes()) {
         c->walk_cfg = cfg;
         st = tvs_run_sectors(c, first, count, TVS_METHOD_BATCHED);   // warm-up
         float total = 0.0f;

@@ -181,6 +181,8 @@ def load() -> ctypes.CDLL:
         "tvs_batch_upload_f32": (st, [vp, ctypes.c_uint32, vp, ctypes.c_size_t]),
         "tvs_batch_compute_bands": (st, [vp]),
         "tvs_batch_set_walk_shape": (st, [vp, P(WalkShape)]),
+        "tvs_batch_autotune": (st, [vp, ctypes.c_uint32, P(WalkShape), ctypes.c_size_t,
+                                    P(ctypes.c_size_t), P(WalkShape)]),
         "tvs_batch_run": (st, [vp, ctypes.c_uint32]),
         "tvs_batch_sync": (st, [vp]),
         "tvs_batch_reset_surfaces": (st, [vp]),
@@ -473,6 +475,17 @@ class Batch:
         self._c(load().tvs_batch_set_walk_shape(self._h, ctypes.byref(
             WalkShape(block, unroll, int(xcd_contiguous), int(pipeline), int(persistent),
                       int(both_directions), 0.0))))
+
+    def autotune(self, reps: int = 1):
+        """Time every compiled launch shape on the whole batch, keep the
+        fastest; cumulative surfaces are unchanged."""
+        n = int(load().tvs_walk_shape_count())
+        res = (WalkShape * n)()
+        got = ctypes.c_size_t(0)
+        best = WalkShape()
+        self._c(load().tvs_batch_autotune(self._h, reps, res, n, ctypes.byref(got),
+                                          ctypes.byref(best)))
+        return best.as_dict(), [res[i].as_dict() for i in range(got.value)]
 
     def run(self, use_graph: bool = True) -> None:
         self._c(load().tvs_batch_run(self._h, int(use_graph)))
