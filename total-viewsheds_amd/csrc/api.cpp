@@ -192,6 +192,14 @@ tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t cap
     const uint32_t N = c->geo.computable_points_count;
     if (!walk_shape_supported(cfg))
         return fail(c, TVS_ERR_INVALID_ARGUMENT, "walk shape not compiled");
+    // Heights are addressed with 32-bit byte offsets from a resource that
+    // starts up to w*w - 1 elements below the DEM (step positions are at
+    // most 4 (w*w - 1) int16 bytes, twice that for f32 heights).
+    const uint64_t w2 = static_cast<uint64_t>(c->geo.dem_width) * c->geo.dem_width;
+    if (w2 > (c->has_i16 ? (1ull << 30) : (1ull << 29)))
+        return fail(c, TVS_ERR_SIZE,
+                    "DEM too wide for the batched walk's 32-bit height offsets (int16 up to "
+                    "32768 points wide, f32 up to 23170); use TVS_METHOD_PLAIN");
     if (c->partial_cap < count) {
         dev_free(c->partial);
         c->partial_cap = 0;

@@ -290,10 +290,12 @@ __global__ __launch_bounds__(kScanThreads) void step_table_kernel(
         tail = max(tail, red_tail[wv]);
     }
     const int32_t bias = max(hi, -lo);
+    // Byte offsets of int16 elements: 2 * (bias +/- off) (the host keeps
+    // them below 2^32).
     for (uint32_t i = begin; i < end; ++i) {
         const int32_t o = tab.off[tb + i];
-        tab.pos_f[tb + i] = static_cast<uint32_t>(bias + o);
-        tab.pos_b[tb + i] = static_cast<uint32_t>(bias - o);
+        tab.pos_f[tb + i] = 2u * static_cast<uint32_t>(bias + o);
+        tab.pos_b[tb + i] = 2u * static_cast<uint32_t>(bias - o);
     }
     if (tid == 0) {
         SectorMeta m{};

@@ -48,8 +48,8 @@ struct StepTables {
     int32_t *delta = nullptr;  // band_deltas            (band_of_sight.rs:165)
     float *dist = nullptr;     // band_distances         (band_of_sight.rs:168)
     int32_t *off = nullptr;    // inclusive prefix sum of delta
-    uint32_t *pos_f = nullptr; // meta.bias + off: forward element position
-    uint32_t *pos_b = nullptr; // meta.bias - off: backward element position
+    uint32_t *pos_f = nullptr; // 2 (meta.bias + off): forward position, int16 bytes
+    uint32_t *pos_b = nullptr; // 2 (meta.bias - off): backward position, int16 bytes
     float *rcp = nullptr;      // RN(1 / dist)
     float *rcp_lo = nullptr;   // RN((1 - dist * rcp) * rcp): 1/dist - rcp
     float *curv = nullptr;     // RN(dist / 12742000)    (kernel.rs:175)

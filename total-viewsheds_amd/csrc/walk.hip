@@ -132,6 +132,13 @@ __device__ __forceinline__ float add_if(bool v, float surface, float w) {
     return __builtin_fmaf(v ? 1.0f : 0.0f, w, surface);
 }
 
+// Scalar byte offset of a step for heights of type H, from its int16 byte
+// position in the step table.
+template <typename H>
+__device__ __forceinline__ uint32_t soff(uint32_t pos16) {
+    return sizeof(H) == 2 ? pos16 : pos16 << 1;
+}
+
 // Buffer resource word 3 (gfx9 family): DST_SEL_X [2:0], NUM_FORMAT
 // [14:12], DATA_FORMAT [18:15].  Untyped loads need a non-zero data format
 // (32); the int16 heights use DST_SEL_X = X (4), NUM_FORMAT = SSCALED (3),
@@ -141,32 +148,30 @@ template <> constexpr int kHeightRsrcFlags<int16_t> = 4 | (3 << 12) | (2 << 15);
 
 // One step's update where the walk records no rings (kernel.rs:216, 233):
 // lanes with angle > max_angle add w to the surface (exec narrowed by
-// v_cmpx, restored at once), then max_angle = max(angle, max_angle) on all
-// lanes.  v_max_f32 equals `angle > max_angle ? angle : max_angle` for the
+// v_cmpx, then set back to `lanes`), then max_angle = max(angle, max_angle)
+// on all lanes.  `lanes` is the exec mask read at the start of the walk: the
+// walk has no divergent control flow, so exec equals it at every step.
+// v_max_f32 equals `angle > max_angle ? angle : max_angle` for the
 // comparisons that follow: max_angle is never NaN, a NaN angle leaves it
 // (quiet NaNs only, from arithmetic), and equal values differ at most in
 // the sign of zero, which no comparison sees.
-__device__ __forceinline__ void step_update(float angle, float w, float &max_angle,
-                                            float &surface) {
-    uint64_t saved_exec;
-    asm("s_mov_b64 %[e], exec\n\t"
-        "v_cmpx_gt_f32 vcc, %[a], %[m]\n\t"
+__device__ __forceinline__ void step_update(float angle, float w, uint64_t lanes,
+                                            float &max_angle, float &surface) {
+    asm("v_cmpx_gt_f32 vcc, %[a], %[m]\n\t"
         "v_add_f32 %[s], %[w], %[s]\n\t"
         "s_mov_b64 exec, %[e]\n\t"
         "v_max_f32 %[m], %[a], %[m]"
-        : [m] "+v"(max_angle), [s] "+v"(surface), [e] "=&s"(saved_exec)
-        : [a] "v"(angle), [w] "s"(w)
+        : [m] "+v"(max_angle), [s] "+v"(surface)
+        : [a] "v"(angle), [w] "s"(w), [e] "s"(lanes)
         : "vcc");
 }
 
 // step_update for a lane's forward (f) and backward (b) band at the same
-// step: one exec save serves both masked adds.
+// step.
 __device__ __forceinline__ void pair_update(float angle_f, float angle_b, float w,
-                                            float &max_f, float &max_b, float &surface_f,
-                                            float &surface_b) {
-    uint64_t saved_exec;
-    asm("s_mov_b64 %[e], exec\n\t"
-        "v_cmpx_gt_f32 vcc, %[af], %[mf]\n\t"
+                                            uint64_t lanes, float &max_f, float &max_b,
+                                            float &surface_f, float &surface_b) {
+    asm("v_cmpx_gt_f32 vcc, %[af], %[mf]\n\t"
         "v_add_f32 %[sf], %[w], %[sf]\n\t"
         "s_mov_b64 exec, %[e]\n\t"
         "v_cmpx_gt_f32 vcc, %[ab], %[mb]\n\t"
@@ -174,9 +179,8 @@ __device__ __forceinline__ void pair_update(float angle_f, float angle_b, float 
         "s_mov_b64 exec, %[e]\n\t"
         "v_max_f32 %[mf], %[af], %[mf]\n\t"
         "v_max_f32 %[mb], %[ab], %[mb]"
-        : [mf] "+v"(max_f), [mb] "+v"(max_b), [sf] "+v"(surface_f), [sb] "+v"(surface_b),
-          [e] "=&s"(saved_exec)
-        : [af] "v"(angle_f), [ab] "v"(angle_b), [w] "s"(w)
+        : [mf] "+v"(max_f), [mb] "+v"(max_b), [sf] "+v"(surface_f), [sb] "+v"(surface_b)
+        : [af] "v"(angle_f), [ab] "v"(angle_b), [w] "s"(w), [e] "s"(lanes)
         : "vcc");
 }
 
@@ -212,10 +216,10 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
         static_cast<int>((bias + d.w * d.w) * sizeof(H)), kHeightRsrcFlags<H>);
 
     const size_t tb = static_cast<size_t>(slot) * d.stride;
-    // Element position of each step relative to the resource base:
-    // bias + off forward, bias - off backward (>= 0, from the step table).
+    // Position of each step relative to the resource base in int16 bytes:
+    // 2 (bias + off) forward, 2 (bias - off) backward (from the step table).
     const TVS_GLOBAL uint32_t *__restrict__ pos = global_ptr((backward ? d.pos_b : d.pos_f) + tb);
-    auto soff = [&](uint32_t p) -> uint32_t { return p * static_cast<uint32_t>(sizeof(H)); };
+    const uint64_t lanes = __builtin_amdgcn_read_exec();
     const TVS_GLOBAL float *__restrict__ dist = global_ptr(d.dist + tb);
     const TVS_GLOBAL float *__restrict__ rcp = global_ptr(d.rcp + tb);
     const TVS_GLOBAL float *__restrict__ rcp_lo = global_ptr(d.rcp_lo + tb);
@@ -250,7 +254,7 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
     }
     auto event = [&](uint32_t p) {
         ++ring_id;
-        const uint32_t id = pov + p - bias;   // pov +/- off
+        const uint32_t id = pov + (p >> 1) - bias;   // pov +/- off
         if (d.ring_mode == kRingRecords) {
             if (ring_id < d.R) band_ring[ring_id] = id; else ++dropped;
         } else if (d.ring_mode == kRingCompact) {
@@ -268,7 +272,7 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
                                  : dh / dist[k0 + u];
             const float angle = q - curv[k0 + u];
             if (!CAPTURE) {
-                step_update(angle, wt[k0 + u], max_angle, surface);
+                step_update(angle, wt[k0 + u], lanes, max_angle, surface);
                 continue;
             }
             const bool v = angle > max_angle;
@@ -284,13 +288,13 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
         if (fast_end) {
             float h[UNROLL];
 #pragma unroll
-            for (int u = 0; u < UNROLL; ++u) h[u] = load_height<H>(rsrc, pov_bytes, soff(pos[u]));
+            for (int u = 0; u < UNROLL; ++u) h[u] = load_height<H>(rsrc, pov_bytes, soff<H>(pos[u]));
             for (; k < fast_end; k += UNROLL) {
                 const uint32_t kn = k + UNROLL < fast_end ? k + UNROLL : k;
                 float hn[UNROLL];
 #pragma unroll
                 for (int u = 0; u < UNROLL; ++u)
-                    hn[u] = load_height<H>(rsrc, pov_bytes, soff(pos[kn + u]));
+                    hn[u] = load_height<H>(rsrc, pov_bytes, soff<H>(pos[kn + u]));
                 chunk(h, k);
 #pragma unroll
                 for (int u = 0; u < UNROLL; ++u) h[u] = hn[u];
@@ -300,14 +304,14 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
         for (; k < fast_end; k += UNROLL) {
             float h[UNROLL];
 #pragma unroll
-            for (int u = 0; u < UNROLL; ++u) h[u] = load_height<H>(rsrc, pov_bytes, soff(pos[k + u]));
+            for (int u = 0; u < UNROLL; ++u) h[u] = load_height<H>(rsrc, pov_bytes, soff<H>(pos[k + u]));
             chunk(h, k);
         }
     }
     for (; k < tail_start; ++k) {
         const uint32_t o = pos[k];
         const float dk = dist[k];
-        const float dh = load_height<H>(rsrc, pov_bytes, soff(o)) - h0;
+        const float dh = load_height<H>(rsrc, pov_bytes, soff<H>(o)) - h0;
         const float q = (FAST && k < n_fast) ? div_by_reciprocal(dh, dk, rcp[k], rcp_lo[k])
                                              : dh / dk;
         const float angle = q - curv[k];
@@ -322,7 +326,7 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
         // angle (dh/0 - 0: +inf, -inf or NaN) at every step, and the weight
         // 0 * 0.0174533 leaves the surface unchanged.
         const uint32_t o = pos[k];
-        const float dh = load_height<H>(rsrc, pov_bytes, soff(o)) - h0;
+        const float dh = load_height<H>(rsrc, pov_bytes, soff<H>(o)) - h0;
         const float angle = dh / dist[k] - curv[k];
 #pragma unroll 8
         for (; k < d.n_steps; ++k) {
@@ -333,7 +337,7 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
         }
     }
     if (CAPTURE && capture) {
-        if (visible) event(d.n_steps ? pos[d.n_steps - 1] : bias);
+        if (visible) event(d.n_steps ? pos[d.n_steps - 1] : 2u * bias);
         if (d.ring_mode == kRingRecords) {
             if (d.R > 0) band_ring[0] = ring_id;
             if (dropped) atomicAdd(d.overflow, static_cast<unsigned long long>(dropped));
@@ -370,7 +374,7 @@ __device__ __forceinline__ void walk_pair(const TileDesc &d, uint32_t rel, uint3
     const size_t tb = static_cast<size_t>(slot) * d.stride;
     const TVS_GLOBAL uint32_t *__restrict__ pos_f = global_ptr(d.pos_f + tb);
     const TVS_GLOBAL uint32_t *__restrict__ pos_b = global_ptr(d.pos_b + tb);
-    auto soff = [&](uint32_t p) -> uint32_t { return p * static_cast<uint32_t>(sizeof(H)); };
+    const uint64_t lanes = __builtin_amdgcn_read_exec();
     const TVS_GLOBAL float *__restrict__ dist = global_ptr(d.dist + tb);
     const TVS_GLOBAL float *__restrict__ rcp = global_ptr(d.rcp + tb);
     const TVS_GLOBAL float *__restrict__ rcp_lo = global_ptr(d.rcp_lo + tb);
@@ -385,8 +389,8 @@ __device__ __forceinline__ void walk_pair(const TileDesc &d, uint32_t rel, uint3
         float hf[UNROLL], hb[UNROLL];
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
-            hf[u] = load_height<H>(rsrc, pov_bytes, soff(pos_f[k + u]));
-            hb[u] = load_height<H>(rsrc, pov_bytes, soff(pos_b[k + u]));
+            hf[u] = load_height<H>(rsrc, pov_bytes, soff<H>(pos_f[k + u]));
+            hb[u] = load_height<H>(rsrc, pov_bytes, soff<H>(pos_b[k + u]));
         }
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
@@ -396,18 +400,19 @@ __device__ __forceinline__ void walk_pair(const TileDesc &d, uint32_t rel, uint3
                                   : dhf / dist[k + u];
             const float qb = FAST ? div_by_reciprocal(dhb, dist[k + u], rcp[k + u], rcp_lo[k + u])
                                   : dhb / dist[k + u];
-            pair_update(qf - curv[k + u], qb - curv[k + u], wt[k + u], max_f, max_b, surface_f,
-                        surface_b);
+            pair_update(qf - curv[k + u], qb - curv[k + u], wt[k + u], lanes, max_f, max_b,
+                        surface_f, surface_b);
         }
     }
     for (; k < tail_start; ++k) {
         const float dk = dist[k];
-        const float dhf = load_height<H>(rsrc, pov_bytes, soff(pos_f[k])) - h0;
-        const float dhb = load_height<H>(rsrc, pov_bytes, soff(pos_b[k])) - h0;
+        const float dhf = load_height<H>(rsrc, pov_bytes, soff<H>(pos_f[k])) - h0;
+        const float dhb = load_height<H>(rsrc, pov_bytes, soff<H>(pos_b[k])) - h0;
         const bool fast_k = FAST && k < n_fast;
         const float qf = fast_k ? div_by_reciprocal(dhf, dk, rcp[k], rcp_lo[k]) : dhf / dk;
         const float qb = fast_k ? div_by_reciprocal(dhb, dk, rcp[k], rcp_lo[k]) : dhb / dk;
-        pair_update(qf - curv[k], qb - curv[k], wt[k], max_f, max_b, surface_f, surface_b);
+        pair_update(qf - curv[k], qb - curv[k], wt[k], lanes, max_f, max_b, surface_f,
+                    surface_b);
     }
     if (active) {
         TVS_GLOBAL float *out = global_ptr(d.partial) + static_cast<size_t>(2u * rel) * d.N + t;
