@@ -8,10 +8,14 @@ HBM before timing starts; band data computation (device sort + compile) is
 timed separately and reported as `bands_ms`.
 
 Prints one JSON line.  `value` is point-steps/s of the batched method
-(int16 heights); the plain one-sector-per-launch FP32 method is measured in
-the same run and reported under `plain`, with the TVS relative error between
-the two.  `cpu_baseline` times the oracle's literal restatement of the
-reference CPU loop on a bounded sample of the same workload.
+(int16 heights) counted in executed steps: the batched walk skips each
+band's zero-delta tail (its angle is the same at every step and its weight
+is +0, so the tail changes no surface), so only the 2B - bs/2 real steps per
+band are counted (`nominal_value` counts all B - 1, as the reference loop
+executes them).  The plain one-sector-per-launch FP32 method executes every
+step and is reported under `plain`, with the TVS relative error between the
+two.  `cpu_baseline` times the oracle's literal restatement of the reference
+CPU loop (every step) on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -30,6 +34,12 @@ sys.path.insert(0, os.path.join(ROOT, "total-viewsheds_amd"))
 METRIC = ("band-of-sight point-steps/s on 1201² DEM; batched vs plain at "
           "≤1e-4 TVS rel err")
 HBM_PEAK_GBPS = 8000.0
+# VALU lane-operations per second: 256 CUs x 64 lanes x 2.4 GHz.
+VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
+# Vector instructions per real band-step of the batched walk's unrolled
+# loop (sub, 4 for the divide, sub, cmpx, add, max; tests/test_codegen.py
+# holds the compiled loop to this count).
+VALU_PER_STEP = 9
 
 
 def log(msg: str) -> None:
@@ -61,7 +71,9 @@ def cpu_baseline(wl, sectors: int, threads: int):
     import oracle as O
     dem = O.dem_new(wl.width, wl.spacing, wl.max_line_of_sight)
     picked = list(range(0, 180, max(1, 180 // sectors)))[:sectors]
+    t0 = time.perf_counter()
     deltas, dists = O.band_data_all(dem, picked)
+    bands_s = time.perf_counter() - t0
     elev = wl.elevations.astype(np.float32)
     t0 = time.perf_counter()
     O.walk_sectors(dem, elev, wl.observer_height, deltas, dists, n_threads=threads)
@@ -78,9 +90,12 @@ def cpu_baseline(wl, sectors: int, threads: int):
         pass
     return {"value": steps / dt, "unit": "point-steps/s", "cores": threads,
             "kind": "port",
+            "bands_ms_per_sector": bands_s * 1000.0 / len(picked),
             "sample": (f"{len(picked)} of 180 sectors (every {180 // len(picked)}th) of "
-                       f"{wl.name}, walk only, oracle C restatement of the reference "
-                       f"CPU loop, {threads} thread(s), {dt:.1f} s on {cpu}")}
+                       f"{wl.name}, walk only (every step, zero tail included), oracle C "
+                       f"restatement of the reference CPU loop, {threads} thread(s), "
+                       f"{dt:.1f} s on {cpu}; band data (axes sorts + compile) timed "
+                       f"separately, one thread")}
 
 
 def measure_large(steps: int, warmup: int) -> dict:
@@ -96,11 +111,14 @@ def measure_large(steps: int, warmup: int) -> dict:
         bands_ms = e.timing().bands_ms
         shape, _ = e.autotune(0, 36, reps=1)
         ms, kernel_ms, tm = time_method(e, "batched", steps, warmup)
-    ps = 180 * 2 * g.computable_points_count * g.band_deltas_size
-    log(f"large: {ms:.2f} ms/step, {ps / ms * 1e3:.3e} point-steps/s")
+    ps = 180 * 2 * g.computable_points_count * g.real_steps
+    nominal = 180 * 2 * g.computable_points_count * g.band_deltas_size
+    log(f"large: {ms:.2f} ms/step, {ps / ms * 1e3:.3e} executed point-steps/s")
     return {"workload": wl.name, "value": ps / (ms / 1000.0), "unit": "point-steps/s",
+            "nominal_value": nominal / (ms / 1000.0),
             "ms_per_step": ms, "kernel_ms": kernel_ms, "reduce_ms": tm.reduce_ms,
-            "point_steps_per_step": ps, "bands_ms": bands_ms,
+            "point_steps_per_step": ps, "nominal_point_steps_per_step": nominal,
+            "bands_ms": bands_ms,
             "achieved_GBps": ps * 2 / (kernel_ms / 1000.0) / 1e9,
             "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous", "pipeline",
                                                  "persistent", "both_directions")}}
@@ -133,14 +151,17 @@ def measure_tiles(steps: int, warmup: int) -> dict:
             graph_ms.append(b.timing().walk_ms)
         b.sync()
         ms = (time.perf_counter() - t0) * 1000.0 / steps
-        ps = b.timing().point_steps
+        nominal = b.timing().point_steps
+        ps = sum(180 * 2 * g.computable_points_count * g.real_steps for g in b.geometry)
         b.run(False)                         # the launch pair without the graph
         tm = b.timing()
-    log(f"tiles-64: {ms:.2f} ms/step, {ps / ms * 1e3:.3e} point-steps/s")
+    log(f"tiles-64: {ms:.2f} ms/step, {ps / ms * 1e3:.3e} executed point-steps/s")
     return {"workload": "tiles-64", "value": ps / (ms / 1000.0), "unit": "point-steps/s",
+            "nominal_value": nominal / (ms / 1000.0),
             "ms_per_step": ms, "graph_ms": float(np.mean(graph_ms)),
             "kernel_ms": tm.walk_ms, "reduce_ms": tm.reduce_ms,
-            "point_steps_per_step": ps, "bands_ms": bands_ms,
+            "point_steps_per_step": ps, "nominal_point_steps_per_step": nominal,
+            "bands_ms": bands_ms,
             "tiles": len(tiles),
             "sides": sorted({t.width for t in tiles}),
             "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous",
@@ -259,6 +280,8 @@ def main() -> int:
     bands_ms = e.timing().bands_ms
     log(f"band data for 180 sectors: {bands_ms:.1f} ms device, {bands_wall_ms:.1f} ms wall, "
         f"{e.bands_sorted_count()} sectors through the sorts")
+    # The plain walk executes every step of every band (the reference's loop);
+    # the batched walk executes the real steps and skips the zero tail.
     point_steps = 180 * 2 * g.computable_points_count * g.band_deltas_size
     real_point_steps = 180 * 2 * g.computable_points_count * g.real_steps
 
@@ -281,16 +304,17 @@ def main() -> int:
     if not args.no_plain:
         log("timing plain (FP32 heights, one sector per launch)")
         p_ms, p_kernel_ms, _ = time_method(e, "plain", args.steps, args.warmup)
-        plain = {"value": point_steps / (p_ms / 1000.0), "ms_per_step": p_ms,
+        plain = {"value": point_steps / (p_ms / 1000.0), "ms_per_step": p_ms,   # every step
                  "kernel_ms": p_kernel_ms, "launches_per_step": 180,
                  "achieved_GBps": point_steps * 4 / (p_kernel_ms / 1000.0) / 1e9}
         log(f"plain: {p_ms:.2f} ms/step, {plain['value']:.3e} point-steps/s")
 
     log("timing batched (int16 heights, all sectors in one launch)")
     b_ms, b_kernel_ms, tm = time_method(e, "batched", args.steps, args.warmup)
-    value = point_steps / (b_ms / 1000.0)
+    value = real_point_steps / (b_ms / 1000.0)
     log(f"batched: {b_ms:.2f} ms/step (walk kernel {b_kernel_ms:.2f} ms, reduce "
-        f"{tm.reduce_ms:.2f} ms), {value:.3e} point-steps/s")
+        f"{tm.reduce_ms:.2f} ms), {value:.3e} executed point-steps/s "
+        f"({point_steps / (b_ms / 1000.0):.3e} nominal)")
 
     batched_f32 = None
     if not args.no_f32:
@@ -302,9 +326,10 @@ def main() -> int:
             ef.set_walk_shape(shape["block"], shape["unroll"], shape["xcd_contiguous"],
                               shape["pipeline"], shape["persistent"], shape["both_directions"])
             f_ms, f_kernel_ms, _ = time_method(ef, "batched", args.steps, args.warmup)
-        batched_f32 = {"value": point_steps / (f_ms / 1000.0), "ms_per_step": f_ms,
+        batched_f32 = {"value": real_point_steps / (f_ms / 1000.0), "ms_per_step": f_ms,
+                       "nominal_value": point_steps / (f_ms / 1000.0),
                        "kernel_ms": f_kernel_ms,
-                       "achieved_GBps": point_steps * 4 / (f_kernel_ms / 1000.0) / 1e9}
+                       "achieved_GBps": real_point_steps * 4 / (f_kernel_ms / 1000.0) / 1e9}
         log(f"batched f32: {f_ms:.2f} ms/step, {batched_f32['value']:.3e} point-steps/s")
 
     # Correctness of the timed configuration: one fresh pass of each method.
@@ -320,8 +345,9 @@ def main() -> int:
     rel_err64 = float(np.max(np.abs(tvs_b64 - tvs_p) / tvs_b64))
     bit_exact = bool(np.array_equal(tvs_b.view(np.uint32), tvs_p.view(np.uint32)))
 
-    bytes_per_launch = point_steps * 2          # one int16 height per point-step
+    bytes_per_launch = real_point_steps * 2     # one int16 height per executed step
     achieved = bytes_per_launch / (b_kernel_ms / 1000.0) / 1e9
+    valu_lane_ops = real_point_steps * VALU_PER_STEP / (b_kernel_ms / 1000.0)
     traffic = None
     prof = os.path.join(ROOT, "profiles", "walk_tiles_main-1201_traffic.json")
     if os.path.exists(prof):
@@ -351,7 +377,8 @@ def main() -> int:
             "sectors": 180,
             "method": "batched",
             "heights": "int16",
-            "point_steps_per_step": point_steps,
+            "point_steps_per_step": real_point_steps,
+            "nominal_point_steps_per_step": point_steps,
             "steps_per_band": g.band_deltas_size,
             "real_steps_per_band": g.real_steps,
             "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous",
@@ -368,7 +395,13 @@ def main() -> int:
             "kernel_ms": b_kernel_ms,
             "bytes_per_launch": bytes_per_launch,
         },
-        "real_point_steps_per_s": real_point_steps / (b_ms / 1000.0),
+        "nominal_value": point_steps / (b_ms / 1000.0),
+        "valu": {
+            "instructions_per_step": VALU_PER_STEP,
+            "achieved_lane_ops_per_s": valu_lane_ops,
+            "peak_lane_ops_per_s": VALU_PEAK_LANE_OPS,
+            "frac": valu_lane_ops / VALU_PEAK_LANE_OPS,
+        },
         "plain": plain,
         "batched_f32": batched_f32,
         "tvs_rel_err_batched_vs_plain": rel_err,
