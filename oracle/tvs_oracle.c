@@ -186,7 +186,9 @@ int tvso_axes_compute(uint32_t width, uint16_t sector, float *distances_f32,
     return st;
 }
 
-static const uint32_t *g_sort_rank; /* qsort has no context argument */
+/* qsort has no context argument; one per thread, so band data of several
+ * sectors can be compiled concurrently. */
+static _Thread_local const uint32_t *g_sort_rank;
 
 static int cmp_by_rank(const void *a, const void *b)
 {

@@ -220,6 +220,19 @@ def test_threaded_walk_matches_serial(oracle):
     np.testing.assert_array_equal(a, b)
 
 
+def test_concurrent_band_data_matches_serial(oracle):
+    """Band data compiled for many sectors from several threads at once (as
+    the bench's all-cores baseline does) equals the serial result."""
+    from concurrent.futures import ThreadPoolExecutor
+    dem = oracle.dem_new(101, 30.0, 1000)
+    d1, x1 = oracle.band_data_all(dem, range(180))
+    with ThreadPoolExecutor(8) as ex:
+        parts = list(ex.map(lambda s: oracle.band_data_all(dem, [s]), range(180)))
+    np.testing.assert_array_equal(np.concatenate([p[0] for p in parts]), d1)
+    np.testing.assert_array_equal(np.concatenate([p[1] for p in parts]).view(np.uint32),
+                                  x1.view(np.uint32))
+
+
 def test_f64_companion(oracle):
     """The f64 walk agrees with the f32 walk to well within 1e-4 and sees
     no visibility flips on a flat DEM."""
