@@ -142,6 +142,19 @@ tvs_status tvs_upload_dem_i16(tvs_ctx *ctx, const int16_t *elevations,
 tvs_status tvs_compute_bands(tvs_ctx *ctx, uint32_t first_sector,
                              uint32_t sector_count);
 
+/* The batched walk's divide.  Regular steps normally use a four-operation
+ * correctly rounded divide (split reciprocal + exact residual step).  With
+ * the short divide on (the default), a sector whose every step gives the
+ * same quotient with a three-operation divide, for every height difference
+ * the uploaded DEM can produce (checked on the device before the first run
+ * after an upload or band change), uses the three-operation one; results
+ * are identical either way.  `verified_sectors` counts the sectors that
+ * passed the last checks.  No reference counterpart (kernel.rs:175 is a
+ * plain f32 divide). */
+tvs_status tvs_set_short_divide(tvs_ctx *ctx, uint32_t enable);
+tvs_status tvs_get_short_divide(tvs_ctx *ctx, uint32_t *enabled,
+                                uint32_t *verified_sectors);
+
 /* How tvs_compute_bands finds each sector's window and its sight order.
  * SELECT (default): per-row binary searches over the monotone rows of the
  * rotated key pick the window exactly, and the samples are ordered by
@@ -297,6 +310,8 @@ tvs_status tvs_batch_upload_f32(tvs_batch *batch, uint32_t tile,
 /* Band data for all 180 sectors of every distinct (width, band size). */
 tvs_status tvs_batch_compute_bands(tvs_batch *batch);
 tvs_status tvs_batch_set_walk_shape(tvs_batch *batch, const tvs_walk_shape *shape);
+/* tvs_set_short_divide for every tile. */
+tvs_status tvs_batch_set_short_divide(tvs_batch *batch, uint32_t enable);
 /* tvs_autotune for a batch: times the walk launch over all tiles in every
  * compiled shape and keeps the fastest.  Only the walk runs, so the tiles'
  * cumulative surfaces are unchanged.  Needs band data and elevations. */

@@ -2,8 +2,8 @@
 hipcc cross-compiles).  Guards the properties the kernel's speed rests on:
 8 height loads with scalar offsets per chunk (typed loads that convert the
 int16 heights, so no conversion instruction), the step tables read with
-scalar dwordx8 loads, the exec-masked visibility update, and no divergent
-branch in the loop."""
+scalar dwordx8 loads, the exec-masked visibility update, one FMA fewer per
+step in the short-divide loop, and no divergent branch in the loop."""
 import os
 import re
 import shutil
@@ -35,44 +35,65 @@ def kernel_body(asm: str, mangled_part: str) -> str:
     return asm[start:asm.index("s_endpgm", start)]
 
 
-def main_loop(body: str) -> str:
-    """The loop whose body holds the unrolled height loads."""
+def unrolled_loops(body: str) -> list[str]:
+    """The loops whose bodies hold the unrolled height loads: one per divide
+    variant (split reciprocal, short, IEEE), in the kernel's order."""
     lines = body.splitlines()
-    heads = [i for i, l in enumerate(lines) if "Inner Loop Header" in l]
-    for h in heads:
-        end = next(i for i in range(h, len(lines)) if re.search(r"s_cbranch_scc\d", lines[i]))
+    out = []
+    for h, line in enumerate(lines):
+        if "Inner Loop Header" not in line:
+            continue
+        end = next((i for i in range(h, len(lines)) if re.search(r"s_cbranch_scc\d", lines[i])),
+                   len(lines) - 1)
         chunk = "\n".join(lines[h:end + 1])
         if len(re.findall(r"buffer_load_(format_x|dword)\s", chunk)) >= 8:
-            return chunk
-    raise AssertionError("no unrolled loop found")
+            out.append(chunk)
+    assert out, "no unrolled loop found"
+    return out
+
+
+def valu(loop: str) -> int:
+    return len(re.findall(r"^\s+v_", loop, re.M))
+
+
+def check_loop(loop: str, load: str, band_steps: int) -> None:
+    assert "s_cbranch_execz" not in loop and "s_cbranch_vccz" not in loop
+    assert len(re.findall(load + r"\s+v\d+, v\d+, s\[\d+:\d+\], s\d+ offen", loop)) == band_steps
+    assert "s_load_dwordx8" in loop
+    assert "v_cvt_f32_i32" not in loop
+    assert len(re.findall(r"v_cmpx_gt_f32", loop)) == band_steps
+    assert "scratch" not in loop and "buffer_store" not in loop
 
 
 @pytest.mark.parametrize("heights,load", [("Is", "buffer_load_format_x"),
                                           ("If", "buffer_load_dword")])
 @pytest.mark.parametrize("pipe", ["Lb0E", "Lb1E"])
 def test_walk_loop_shape(asm, heights, load, pipe):
-    loop = main_loop(kernel_body(asm, heights + "Lb0ELi256ELi8E" + pipe + "Lb0ELb0E"))
-    assert "s_cbranch_execz" not in loop and "s_cbranch_vccz" not in loop
-    assert len(re.findall(load + r"\s+v\d+, v\d+, s\[\d+:\d+\], s\d+ offen", loop)) == 8
-    assert "s_load_dwordx8" in loop
-    assert "v_cvt_f32_i32" not in loop
-    assert len(re.findall(r"v_cmpx_gt_f32", loop)) == 8
-    valu = len(re.findall(r"^\s+v_", loop, re.M))
-    assert valu <= 9 * 8 + 8, valu                 # 9 per step (+ pipeline moves)
-    assert "scratch" not in loop and "buffer_store" not in loop
+    """One band per lane, 8 steps per chunk: the split-reciprocal loop spends
+    9 VALU per step, the short-divide loop 8, the IEEE loop more."""
+    loops = unrolled_loops(kernel_body(asm, heights + "Lb0ELi256ELi8E" + pipe + "Lb0ELb0E"))
+    fast = [l for l in loops if "v_div_scale" not in l]
+    assert len(fast) == 2
+    for loop in fast:
+        check_loop(loop, load, 8)
+    counts = sorted(valu(l) for l in fast)
+    assert counts[0] <= 8 * 8 + 8 and counts[1] <= 9 * 8 + 8, counts
+    assert counts[1] - counts[0] >= 8                    # one FMA fewer per step
 
 
 def test_paired_walk_loop_shape(asm):
     """Both bands of a point per lane: 16 height loads and 16 masked updates
-    per 8-step chunk, still 9 VALU per band-step, one set of table reads."""
-    loop = main_loop(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb1E"))
-    assert "s_cbranch_execz" not in loop and "s_cbranch_vccz" not in loop
-    assert len(re.findall(r"buffer_load_format_x\s+v\d+, v\d+, s\[\d+:\d+\], s\d+ offen", loop)) == 16
-    assert len(re.findall(r"v_cmpx_gt_f32", loop)) == 16
-    assert len(re.findall(r"s_load_dwordx8", loop)) == 7     # pos_f, pos_b + 5 shared tables
-    valu = len(re.findall(r"^\s+v_", loop, re.M))
-    assert valu <= 9 * 16 + 8, valu
-    assert "scratch" not in loop and "buffer_store" not in loop
+    per 8-step chunk, one set of table reads (pos_f, pos_b + the shared
+    constants), 9 (split reciprocal) or 8 (short divide) VALU per band-step."""
+    loops = unrolled_loops(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb1E"))
+    fast = [l for l in loops if "v_div_scale" not in l]
+    assert len(fast) == 2
+    for loop in fast:
+        check_loop(loop, "buffer_load_format_x", 16)
+    short, split = sorted(fast, key=valu)
+    assert valu(short) <= 8 * 16 + 8 and valu(split) <= 9 * 16 + 8, (valu(short), valu(split))
+    assert len(re.findall(r"s_load_dwordx8", split)) == 7
+    assert len(re.findall(r"s_load_dwordx8", short)) == 6      # no low reciprocal part
 
 
 def test_no_packed_fp32_and_no_spills(asm):

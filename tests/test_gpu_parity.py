@@ -113,6 +113,47 @@ def test_full_int16_range_bit_exact(oracle):
                                           err_msg=np.dtype(dtype).name)
 
 
+@pytest.mark.parametrize("pair", [False, True])
+def test_short_divide_verified_and_exact(small, pair):
+    """The three-operation divide is verified for every sector of this DEM
+    and gives the oracle's surfaces; switched off, the split-reciprocal
+    divide gives the same bits."""
+    wl, dem, _, _, cum = small
+    with make_engine(wl) as e:
+        e.upload(wl.elevations)
+        e.compute_bands(0, 180)
+        e.set_walk_shape(128, 8, True, both_directions=pair)
+        e.run(0, 180, "batched")
+        assert e.short_divide() == (True, 180)
+        np.testing.assert_array_equal(e.surfaces().view(np.uint32), cum.view(np.uint32))
+        e.set_short_divide(False)
+        e.reset_surfaces()
+        e.run(0, 180, "batched")
+        assert e.short_divide() == (False, 0)
+        np.testing.assert_array_equal(e.surfaces().view(np.uint32), cum.view(np.uint32))
+
+
+def test_short_divide_reverified_after_changes(small):
+    """A new upload or new band data clears the verification; the next run
+    checks again (sector count back to 180) and stays exact."""
+    wl, dem, deltas, dists, cum = small
+    with make_engine(wl) as e:
+        e.upload(wl.elevations)
+        e.compute_bands(0, 180)
+        e.run(0, 180, "batched")
+        assert e.short_divide()[1] == 180
+        e.upload(wl.elevations.astype(np.float32))      # integral f32: still eligible
+        assert e.short_divide()[1] == 0
+        e.set_band(7, deltas[7], dists[7])
+        e.reset_surfaces()
+        e.run(0, 180, "batched")
+        assert e.short_divide()[1] == 180
+        np.testing.assert_array_equal(e.surfaces(), cum)
+        e.upload(wl.elevations.astype(np.float32) + np.float32(0.25))  # not integral
+        e.run(0, 180, "batched")
+        assert e.short_divide()[1] == 0
+
+
 def test_small_tvs_from_host_band_data(small):
     """tvs_set_band (band data from outside) feeds the same walk."""
     wl, dem, deltas, dists, cum = small
@@ -236,6 +277,13 @@ def test_main_full_run_methods_agree(main):
     np.testing.assert_array_equal(batched.view(np.uint32), plain.view(np.uint32))
     assert np.all(batched > 0)
     assert np.max(np.abs(b64 - batched) / b64) < 1e-5
+    assert e.short_divide() == (True, 180)      # every sector on the short divide
+    note("main-1201 full batched, split-reciprocal divide")
+    e.set_short_divide(False)
+    e.reset_surfaces()
+    e.run(0, 180, "batched")
+    np.testing.assert_array_equal(e.surfaces().view(np.uint32), plain.view(np.uint32))
+    e.set_short_divide(True)
 
 
 def test_main_rings_one_sector(main, oracle):

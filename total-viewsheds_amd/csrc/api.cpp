@@ -158,12 +158,56 @@ tvs_status ensure_ring(tvs_ctx *c, size_t slots) {
     return TVS_OK;
 }
 
+// Every height difference fl32(h - fl32(hp + obs)) for integer h, hp in
+// [lo, hi] (a superset of what the walk can meet).  h0 = fl32(hp + obs)
+// equals hp + o with o taking a few values (one per binade of hp + obs), and
+// h - h0 = (h - hp) - o exactly, so the set is {fl32(D - o)} over the
+// integer differences D in [lo - hi, hi - lo] and the distinct o; each
+// value is computed the way the walk computes it, from one hp per o.
+std::vector<float> height_differences(int64_t lo, int64_t hi, float obs) {
+    std::vector<int64_t> reps;
+    std::vector<double> offsets;
+    for (int64_t hp = lo; hp <= hi; ++hp) {
+        const float h0 = static_cast<float>(hp) + obs;
+        const double o = static_cast<double>(h0) - static_cast<double>(hp);
+        if (std::find(offsets.begin(), offsets.end(), o) == offsets.end()) {
+            offsets.push_back(o);
+            reps.push_back(hp);
+        }
+    }
+    std::vector<float> out;
+    out.reserve(reps.size() * static_cast<size_t>(2 * (hi - lo) + 1));
+    for (int64_t hp : reps) {
+        const float h0 = static_cast<float>(hp) + obs;
+        for (int64_t dd = lo - hi; dd <= hi - lo; ++dd)
+            out.push_back(static_cast<float>(hp + dd) - h0);
+    }
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+    return out;
+}
+
+// Largest domain the short-divide check takes (values); beyond it the
+// split-reciprocal divide is used everywhere.
+constexpr size_t kMaxDivDomain = size_t(1) << 22;
+
 tvs_status upload_common(tvs_ctx *c, bool integral, int64_t lo, int64_t hi) {
     float mn = 0.0f, mx = 0.0f;
     c->fast_div_ok = integral &&
                      fast_divide_holds(lo, hi, c->params.observer_height, &mn, &mx);
     c->min_abs_dh = mn;
     c->max_abs_dh = mx;
+    std::fill(c->div3_checked.begin(), c->div3_checked.end(), 0);
+    dev_free(c->div_domain);
+    c->div_domain_n = 0;
+    if (!c->fast_div_ok) return TVS_OK;
+    const std::vector<float> dom = height_differences(lo, hi, c->params.observer_height);
+    if (dom.empty() || dom.size() > kMaxDivDomain) return TVS_OK;
+    TVS_HIP(c, dev_alloc(&c->div_domain, dom.size()), "height differences");
+    TVS_HIP(c, hipMemcpy(c->div_domain, dom.data(), dom.size() * sizeof(float),
+                         hipMemcpyHostToDevice),
+            "height differences");
+    c->div_domain_n = static_cast<uint32_t>(dom.size());
     return TVS_OK;
 }
 
@@ -181,6 +225,29 @@ tvs_status validate_slots(tvs_ctx *c, uint32_t first, uint32_t count) {
             return st;
         }
         c->band_ready[s] = 1;
+    }
+    return TVS_OK;
+}
+
+// Verifies the short divide for the unchecked slots of [first, first +
+// count) of context c against step tables `tab` (c's own, or the band
+// owner's in a batch).  No-op when the short divide is off or unavailable.
+tvs_status ensure_div3(tvs_ctx *c, const StepTables &tab, uint32_t first, uint32_t count) {
+    if (!c->short_div || !c->fast_div_ok || !c->div_domain) return TVS_OK;
+    uint32_t s = first;
+    while (s < first + count) {
+        if (c->div3_checked[s]) {
+            ++s;
+            continue;
+        }
+        uint32_t e = s;
+        while (e < first + count && !c->div3_checked[e]) ++e;
+        TVS_HIP(c, launch_fill_u32(c->stream, c->div3_bad + s, e - s, 0u), "divide check");
+        TVS_HIP(c, launch_div3_check(c->stream, tab, s, e - s, c->div_domain,
+                                     c->div_domain_n, c->div3_bad),
+                "divide check");
+        for (uint32_t k = s; k < e; ++k) c->div3_checked[k] = 1;
+        s = e;
     }
     return TVS_OK;
 }
@@ -237,6 +304,7 @@ tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t cap
     d.R = c->rings_per_band;
     d.capture_slots = capture;
     d.fast = c->fast_div_ok ? 1u : 0u;
+    d.div3_bad = (c->short_div && c->fast_div_ok && c->div_domain) ? c->div3_bad : nullptr;
     d.ring_mode = kRingRecords;
     d.ring_counts = nullptr;
     d.ring_offsets = nullptr;
@@ -329,6 +397,7 @@ tvs_status tvs_create(const tvs_params *p, tvs_ctx **out) {
     if (e == hipSuccess) e = dev_alloc(&c->tab.curv, tab_n);
     if (e == hipSuccess) e = dev_alloc(&c->tab.wt, tab_n);
     if (e == hipSuccess) e = dev_alloc(&c->tab.meta, tvs::kBandSlots);
+    if (e == hipSuccess) e = dev_alloc(&c->div3_bad, tvs::kBandSlots);
     if (e == hipSuccess) e = hipMemsetAsync(c->cum, 0, sizeof(float) * std::max(N, 1u), c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->cum64, 0, sizeof(double) * std::max(N, 1u), c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->overflow, 0, sizeof(unsigned long long), c->stream);
@@ -339,6 +408,7 @@ tvs_status tvs_create(const tvs_params *p, tvs_ctx **out) {
         return s2;
     }
     c->band_ready.assign(tvs::kBandSlots, 0);
+    c->div3_checked.assign(tvs::kBandSlots, 0);
     c->meta_host.assign(tvs::kBandSlots, tvs::SectorMeta{});
     *out = c;
     return TVS_OK;
@@ -359,6 +429,8 @@ tvs_status tvs_destroy(tvs_ctx *c) {
     dev_free(c->tab.curv);
     dev_free(c->tab.wt);
     dev_free(c->tab.meta);
+    dev_free(c->div3_bad);
+    dev_free(c->div_domain);
     dev_free(c->cum);
     dev_free(c->cum64);
     dev_free(c->partial);
@@ -464,12 +536,38 @@ tvs_status tvs_compute_bands(tvs_ctx *c, uint32_t first, uint32_t count) {
     }
     c->bands_sorted_sectors = static_cast<uint32_t>(sorted_path.size());
     TVS_HIP(c, tvs::launch_step_tables(c, first, count), "step tables");
+    std::fill(c->div3_checked.begin() + first, c->div3_checked.begin() + first + count, 0);
     TVS_HIP(c, hipEventRecord(c->ev[5], c->stream), "event");
     TVS_HIP(c, hipEventSynchronize(c->ev[5]), "band computation");
     float ms = 0.0f;
     TVS_HIP(c, hipEventElapsedTime(&ms, c->ev[4], c->ev[5]), "event time");
     c->timing.bands_ms = ms;
     return read_meta(c, first, count);
+}
+
+tvs_status tvs_set_short_divide(tvs_ctx *c, uint32_t enable) {
+    if (!c) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null context");
+    c->short_div = enable != 0;
+    return TVS_OK;
+}
+
+tvs_status tvs_get_short_divide(tvs_ctx *c, uint32_t *enabled, uint32_t *verified_sectors) {
+    if (!c) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null context");
+    if (enabled) *enabled = c->short_div ? 1u : 0u;
+    if (verified_sectors) {
+        uint32_t n = 0;
+        if (c->short_div && c->fast_div_ok && c->div_domain) {
+            std::vector<uint32_t> bad(tvs::kSectorSteps, 1u);
+            TVS_HIP(c, hipStreamSynchronize(c->stream), "sync");
+            TVS_HIP(c, hipMemcpy(bad.data(), c->div3_bad, bad.size() * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost),
+                    "divide check readback");
+            for (uint32_t s = 0; s < tvs::kSectorSteps; ++s)
+                if (c->div3_checked[s] && bad[s] == 0u) ++n;
+        }
+        *verified_sectors = n;
+    }
+    return TVS_OK;
 }
 
 tvs_status tvs_set_band_method(tvs_ctx *c, tvs_band_method m) {
@@ -527,6 +625,7 @@ static tvs_status set_band_slot(tvs_ctx *c, uint32_t slot, const int32_t *deltas
                 "band upload");
     }
     TVS_HIP(c, tvs::launch_step_tables(c, slot, 1), "step tables");
+    c->div3_checked[slot] = 0;
     return read_meta(c, slot, 1);
 }
 
@@ -599,6 +698,7 @@ tvs_status tvs_run_sectors(tvs_ctx *c, uint32_t first, uint32_t count, tvs_metho
         tvs::TileDesc d;
         if ((st = tvs::prepare_tile(c, first, count, capture, c->walk_cfg, &d)) != TVS_OK)
             return st;
+        if ((st = tvs::ensure_div3(c, c->tab, first, count)) != TVS_OK) return st;
         const uint32_t n_items = d.blocks_per_pass * 2u * count;
         const uint32_t begin[4] = {0u, n_items, 0u, N};
         TVS_HIP(c, hipMemcpyAsync(c->d_tile, &d, sizeof d, hipMemcpyHostToDevice, c->stream),

@@ -19,6 +19,7 @@
 namespace tvs {
 tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t capture,
                         const WalkConfig &cfg, TileDesc *out);
+tvs_status ensure_div3(tvs_ctx *c, const StepTables &tab, uint32_t first, uint32_t count);
 }
 
 struct tvs_batch {
@@ -96,6 +97,9 @@ tvs_status prepare(tvs_batch *b) {
             if (!c->band_ready[s]) return bfail(b, TVS_ERR_STATE, "tile " + std::to_string(i) + " has no band data");
         tvs::TileDesc d;
         tvs_status st = tvs::prepare_tile(c, 0, tvs::kSectorSteps, 0, b->cfg, &d);
+        if (st != TVS_OK) return from_tile(b, i, st);
+        // The short divide is checked against the tables the tile walks with.
+        st = tvs::ensure_div3(c, o->tab, 0, tvs::kSectorSteps);
         if (st != TVS_OK) return from_tile(b, i, st);
         d.off = o->tab.off;
         d.pos_f = o->tab.pos_f;
@@ -235,8 +239,9 @@ tvs_status tvs_batch_compute_bands(tvs_batch *b) {
     }
     for (uint32_t i = 0; i < b->tiles.size(); ++i) {
         const uint32_t o = b->owner[i];
-        if (o == i) continue;
         tvs_ctx *c = b->tiles[i];
+        std::fill(c->div3_checked.begin(), c->div3_checked.end(), 0);   // new tables
+        if (o == i) continue;
         c->meta_host = b->tiles[o]->meta_host;
         const tvs_status st = tvs::validate_slots(c, 0, tvs::kSectorSteps);
         if (st != TVS_OK) return from_tile(b, i, st);
@@ -311,6 +316,16 @@ tvs_status tvs_batch_autotune(tvs_batch *b, uint32_t reps, tvs_walk_shape *resul
         *best = tvs_walk_shape{winner.block, winner.unroll, winner.xcd_contiguous ? 1u : 0u,
                                winner.pipeline ? 1u : 0u, winner.persistent ? 1u : 0u,
                                winner.pair ? 1u : 0u, winner_ms};
+    return TVS_OK;
+}
+
+tvs_status tvs_batch_set_short_divide(tvs_batch *b, uint32_t enable) {
+    if (!b) return bfail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null batch");
+    for (uint32_t i = 0; i < b->tiles.size(); ++i) {
+        const tvs_status st = tvs_set_short_divide(b->tiles[i], enable);
+        if (st != TVS_OK) return from_tile(b, i, st);
+    }
+    drop_graph(b);
     return TVS_OK;
 }
 

@@ -83,6 +83,8 @@ struct TileDesc {
     uint32_t blocks_per_pass;    // ceil(N / block)
     uint32_t R, capture_slots;
     uint32_t fast;               // heights passed the fast-divide range check
+    const uint32_t *div3_bad;    // per slot: 0 = the short divide was verified for
+                                 // every height difference of this DEM (null: off)
     uint32_t ring_mode;          // kRingRecords / kRingCount / kRingCompact
     uint32_t *ring_counts;       // u32[2N]: ring_id per band (kRingCount)
     const uint64_t *ring_offsets;  // u64[2N]: first event of each band (kRingCompact)
@@ -147,6 +149,13 @@ struct tvs_ctx {
     // Height-difference bounds proven on the host for the fast divide.
     bool fast_div_ok = false;
     float min_abs_dh = 0.0f, max_abs_dh = 0.0f;
+    // Short divide: every height difference the DEM can produce (device),
+    // per-slot verification results and which slots are verified.
+    bool short_div = true;                 // caller's switch (default on)
+    float *div_domain = nullptr;
+    uint32_t div_domain_n = 0;
+    uint32_t *div3_bad = nullptr;          // u32[kSlots]
+    std::vector<uint8_t> div3_checked;     // per slot
 
     tvs::StepTables tab;
     std::vector<uint8_t> band_ready;   // per slot: band data resident
@@ -190,6 +199,12 @@ uint32_t band_select_capacity();
 // ---- launchers (walk.hip) ----
 hipError_t launch_walk_plain(tvs_ctx *c, uint32_t slot, uint32_t *ring_out);
 bool walk_shape_supported(const WalkConfig &cfg);
+// Per slot in [first, first + count): bad[slot] = 1 if the short divide
+// differs from the split-reciprocal divide for any value of `domain` at any
+// regular step of the slot's table (bad must be zero on entry).
+hipError_t launch_div3_check(hipStream_t s, const StepTables &tab, uint32_t first,
+                             uint32_t count, const float *domain, uint32_t n_domain,
+                             uint32_t *bad);
 // Every compiled shape the tuners time: both item mappings, persistent
 // shapes once.
 std::vector<WalkConfig> compiled_walk_shapes();

@@ -43,6 +43,17 @@ TVS_HD float div_by_reciprocal(float a, float d, float y_hi, float y_lo) {
     return std::fma(r, y_hi, q0);
 }
 
+// Three operations: q0 = RN(a*y_hi), then the same exact-residual step.
+// Not proven correctly rounded for every (a, d) -- q0 may miss a/d by more
+// than one ulp -- so the walk uses it for a sector only after the device has
+// compared it with div_by_reciprocal for every height difference the DEM
+// can produce at every regular step of that sector.
+TVS_HD float div_short(float a, float d, float y_hi) {
+    const float q0 = a * y_hi;
+    const float r = std::fma(-q0, d, a);
+    return std::fma(r, y_hi, q0);
+}
+
 // Distances the fast divide is proven for.
 TVS_HD bool regular_distance(float d) { return d >= 0x1p-60f && d <= 0x1p60f; }
 

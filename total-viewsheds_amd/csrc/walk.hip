@@ -197,10 +197,19 @@ __device__ __forceinline__ TVS_GLOBAL T *global_ptr(T *p) {
     return (TVS_GLOBAL T *)p;
 }
 
+// The divide of a regular step.  DIV 4: the split-reciprocal divide
+// (proven correctly rounded in the fast-divide range); DIV 3: the short
+// divide (used only for sectors verified against DIV 4); DIV 0: IEEE.
+template <int DIV>
+__device__ __forceinline__ float step_div(float a, float d, float y_hi, float y_lo) {
+    if (DIV == 4) return div_by_reciprocal(a, d, y_hi, y_lo);
+    if (DIV == 3) return div_short(a, d, y_hi);
+    return a / d;
+}
+
 // One band per lane: (tile d, sector slot d.first_slot + rel, direction)
-// for point t.  FAST selects the split-reciprocal divide for the regular
-// steps (the tile's heights passed the host's range check).
-template <typename H, bool FAST, bool CAPTURE, int UNROLL, bool PIPE>
+// for point t.  DIV picks the divide of the regular steps (see step_div).
+template <typename H, int DIV, bool CAPTURE, int UNROLL, bool PIPE>
 __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool backward,
                                           uint32_t t, bool active) {
     const H *elev = static_cast<const H *>(d.elev);
@@ -268,8 +277,7 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
             const float dh = h[u] - h0;
-            const float q = FAST ? div_by_reciprocal(dh, dist[k0 + u], rcp[k0 + u], rcp_lo[k0 + u])
-                                 : dh / dist[k0 + u];
+            const float q = step_div<DIV>(dh, dist[k0 + u], rcp[k0 + u], rcp_lo[k0 + u]);
             const float angle = q - curv[k0 + u];
             if (!CAPTURE) {
                 step_update(angle, wt[k0 + u], lanes, max_angle, surface);
@@ -312,8 +320,8 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
         const uint32_t o = pos[k];
         const float dk = dist[k];
         const float dh = load_height<H>(rsrc, pov_bytes, soff<H>(o)) - h0;
-        const float q = (FAST && k < n_fast) ? div_by_reciprocal(dh, dk, rcp[k], rcp_lo[k])
-                                             : dh / dk;
+        const float q = (DIV != 0 && k < n_fast) ? step_div<DIV>(dh, dk, rcp[k], rcp_lo[k])
+                                                 : dh / dk;
         const float angle = q - curv[k];
         const bool v = angle > max_angle;
         if (CAPTURE && capture && v != visible) event(o);
@@ -356,7 +364,7 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
 // bookkeeping serves two bands, and the lane carries two independent
 // max-scan chains.  The zero tail is skipped: its angle is the same at every
 // step and its weight is +0, so it changes neither surface.
-template <typename H, bool FAST, int UNROLL>
+template <typename H, int DIV, int UNROLL>
 __device__ __forceinline__ void walk_pair(const TileDesc &d, uint32_t rel, uint32_t t,
                                           bool active) {
     const H *elev = static_cast<const H *>(d.elev);
@@ -396,10 +404,8 @@ __device__ __forceinline__ void walk_pair(const TileDesc &d, uint32_t rel, uint3
         for (int u = 0; u < UNROLL; ++u) {
             const float dhf = hf[u] - h0;
             const float dhb = hb[u] - h0;
-            const float qf = FAST ? div_by_reciprocal(dhf, dist[k + u], rcp[k + u], rcp_lo[k + u])
-                                  : dhf / dist[k + u];
-            const float qb = FAST ? div_by_reciprocal(dhb, dist[k + u], rcp[k + u], rcp_lo[k + u])
-                                  : dhb / dist[k + u];
+            const float qf = step_div<DIV>(dhf, dist[k + u], rcp[k + u], rcp_lo[k + u]);
+            const float qb = step_div<DIV>(dhb, dist[k + u], rcp[k + u], rcp_lo[k + u]);
             pair_update(qf - curv[k + u], qb - curv[k + u], wt[k + u], lanes, max_f, max_b,
                         surface_f, surface_b);
         }
@@ -408,9 +414,9 @@ __device__ __forceinline__ void walk_pair(const TileDesc &d, uint32_t rel, uint3
         const float dk = dist[k];
         const float dhf = load_height<H>(rsrc, pov_bytes, soff<H>(pos_f[k])) - h0;
         const float dhb = load_height<H>(rsrc, pov_bytes, soff<H>(pos_b[k])) - h0;
-        const bool fast_k = FAST && k < n_fast;
-        const float qf = fast_k ? div_by_reciprocal(dhf, dk, rcp[k], rcp_lo[k]) : dhf / dk;
-        const float qb = fast_k ? div_by_reciprocal(dhb, dk, rcp[k], rcp_lo[k]) : dhb / dk;
+        const bool fast_k = DIV != 0 && k < n_fast;
+        const float qf = fast_k ? step_div<DIV>(dhf, dk, rcp[k], rcp_lo[k]) : dhf / dk;
+        const float qb = fast_k ? step_div<DIV>(dhb, dk, rcp[k], rcp_lo[k]) : dhb / dk;
         pair_update(qf - curv[k], qb - curv[k], wt[k], lanes, max_f, max_b, surface_f,
                     surface_b);
     }
@@ -432,6 +438,15 @@ __device__ __forceinline__ uint32_t find_tile(const uint32_t *__restrict__ begin
     return lo;
 }
 
+// Divide of a sector's regular steps (wave-uniform): the short divide where
+// it was verified for this DEM, else the split-reciprocal divide where the
+// heights passed the range check, else IEEE.
+__device__ __forceinline__ int divide_of(const TileDesc &d, uint32_t rel) {
+    if (!d.fast) return 0;
+    if (d.div3_bad && global_ptr(d.div3_bad)[d.first_slot + rel] == 0u) return 3;
+    return 4;
+}
+
 // A work item is (tile, sector, direction, BLOCK points).  With PAIR an item
 // is (tile, sector, BLOCK points), both directions: `item` counts pairs and
 // 2*item is the first of the pair's two single-direction items (every tile
@@ -450,10 +465,13 @@ __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
         const uint32_t t_raw = ob * BLOCK + threadIdx.x;
         const bool active = t_raw < d.N;
         const uint32_t t = active ? t_raw : d.N - 1u;
-        if (d.fast)
-            walk_pair<H, true, UNROLL>(d, rel, t, active);
+        const int div = divide_of(d, rel);
+        if (div == 3)
+            walk_pair<H, 3, UNROLL>(d, rel, t, active);
+        else if (div == 4)
+            walk_pair<H, 4, UNROLL>(d, rel, t, active);
         else
-            walk_pair<H, false, UNROLL>(d, rel, t, active);
+            walk_pair<H, 0, UNROLL>(d, rel, t, active);
         return;
     }
     const uint32_t ti = n_tiles > 1 ? find_tile(item_begin, n_tiles, item) : 0u;
@@ -464,10 +482,15 @@ __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
     const uint32_t t_raw = ob * BLOCK + threadIdx.x;
     const bool active = t_raw < d.N;
     const uint32_t t = active ? t_raw : d.N - 1u;       // spare lanes shadow
-    if (d.fast)
-        walk_band<H, true, CAPTURE, UNROLL, PIPE>(d, pass >> 1, (pass & 1u) != 0, t, active);
+    const uint32_t rel = pass >> 1;
+    const bool backward = (pass & 1u) != 0;
+    const int div = divide_of(d, rel);
+    if (div == 3)
+        walk_band<H, 3, CAPTURE, UNROLL, PIPE>(d, rel, backward, t, active);
+    else if (div == 4)
+        walk_band<H, 4, CAPTURE, UNROLL, PIPE>(d, rel, backward, t, active);
     else
-        walk_band<H, false, CAPTURE, UNROLL, PIPE>(d, pass >> 1, (pass & 1u) != 0, t, active);
+        walk_band<H, 0, CAPTURE, UNROLL, PIPE>(d, rel, backward, t, active);
 }
 
 // One work item per block, or (PERSIST) a grid of a few blocks per CU
@@ -527,6 +550,41 @@ __global__ void widen_i16_kernel(const int16_t *in, float *out, size_t n) {
     for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
          i += static_cast<size_t>(gridDim.x) * blockDim.x)
         out[i] = static_cast<float>(in[i]);
+}
+
+// Short-divide verification: block (slot, 64-step chunk of the regular
+// steps); threads stride over the domain of height differences and compare
+// the short divide with the split-reciprocal divide at each of the chunk's
+// steps.  Any difference marks the slot (every writer stores 1).
+constexpr uint32_t kDiv3Chunk = 64;
+
+__global__ __launch_bounds__(kBlock) void div3_check_kernel(StepTables tab, uint32_t first,
+                                                            const float *__restrict__ domain,
+                                                            uint32_t n_domain,
+                                                            uint32_t *__restrict__ bad) {
+    const uint32_t slot = first + blockIdx.x;
+    const uint32_t n_fast = tab.meta[slot].n_fast;
+    const uint32_t k0 = blockIdx.y * kDiv3Chunk;
+    if (k0 >= n_fast) return;   // uniform over the block
+    const uint32_t nk = min(n_fast - k0, kDiv3Chunk);
+    __shared__ float sd[kDiv3Chunk], shi[kDiv3Chunk], slo[kDiv3Chunk];
+    const size_t tb = static_cast<size_t>(slot) * tab.stride + k0;
+    for (uint32_t k = threadIdx.x; k < nk; k += blockDim.x) {
+        sd[k] = tab.dist[tb + k];
+        shi[k] = tab.rcp[tb + k];
+        slo[k] = tab.rcp_lo[tb + k];
+    }
+    __syncthreads();
+    uint32_t differ = 0;
+    for (uint32_t i = threadIdx.x; i < n_domain; i += blockDim.x) {
+        const float a = domain[i];
+        for (uint32_t k = 0; k < nk; ++k) {
+            const float q4 = div_by_reciprocal(a, sd[k], shi[k], slo[k]);
+            const float q3 = div_short(a, sd[k], shi[k]);
+            differ |= __float_as_uint(q4) ^ __float_as_uint(q3);
+        }
+    }
+    if (differ) bad[slot] = 1u;
 }
 
 // Blocks of a persistent launch: enough for eight waves per SIMD.
@@ -606,6 +664,17 @@ hipError_t launch_walk_tiles(hipStream_t s, const TileDesc *tiles, const uint32_
     if (n_items == 0) return hipSuccess;
     return i16 ? launch_tiles_h<int16_t>(s, tiles, item_begin, n_tiles, n_items, capture, cfg)
                : launch_tiles_h<float>(s, tiles, item_begin, n_tiles, n_items, capture, cfg);
+}
+
+hipError_t launch_div3_check(hipStream_t s, const StepTables &tab, uint32_t first,
+                             uint32_t count, const float *domain, uint32_t n_domain,
+                             uint32_t *bad) {
+    if (count == 0 || n_domain == 0) return hipSuccess;
+    const uint32_t chunks = (tab.stride + kDiv3Chunk - 1) / kDiv3Chunk;
+    if (chunks == 0) return hipSuccess;
+    hipLaunchKernelGGL(div3_check_kernel, dim3(count, chunks), dim3(kBlock), 0, s, tab, first,
+                       domain, n_domain, bad);
+    return hipGetLastError();
 }
 
 hipError_t launch_reduce_tiles(hipStream_t s, const TileDesc *tiles,

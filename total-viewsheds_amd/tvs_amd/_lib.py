@@ -168,6 +168,9 @@ def load() -> ctypes.CDLL:
         "tvs_compact_rings": (st, [vp, ctypes.c_uint32, vp, ctypes.c_size_t, vp,
                                    ctypes.c_size_t, P(ctypes.c_uint64)]),
         "tvs_set_band_method": (st, [vp, ctypes.c_int]),
+        "tvs_set_short_divide": (st, [vp, ctypes.c_uint32]),
+        "tvs_get_short_divide": (st, [vp, P(ctypes.c_uint32), P(ctypes.c_uint32)]),
+        "tvs_batch_set_short_divide": (st, [vp, ctypes.c_uint32]),
         "tvs_bands_sorted_count": (st, [vp, P(ctypes.c_uint32)]),
         "tvs_set_walk_shape": (st, [vp, P(WalkShape)]),
         "tvs_get_walk_shape": (st, [vp, P(WalkShape)]),
@@ -391,6 +394,15 @@ class Engine:
         self._c(load().tvs_get_walk_shape(self._h, ctypes.byref(w)))
         return w.as_dict()
 
+    def set_short_divide(self, enable: bool) -> None:
+        self._c(load().tvs_set_short_divide(self._h, int(enable)))
+
+    def short_divide(self) -> tuple[bool, int]:
+        """(enabled, sectors verified for the three-operation divide)."""
+        en, n = ctypes.c_uint32(0), ctypes.c_uint32(0)
+        self._c(load().tvs_get_short_divide(self._h, ctypes.byref(en), ctypes.byref(n)))
+        return bool(en.value), int(n.value)
+
     def autotune(self, first: int = 0, count: int = 180, reps: int = 2):
         """Time every compiled launch shape on this DEM, keep the fastest."""
         n = int(load().tvs_walk_shape_count())
@@ -475,6 +487,9 @@ class Batch:
         self._c(load().tvs_batch_set_walk_shape(self._h, ctypes.byref(
             WalkShape(block, unroll, int(xcd_contiguous), int(pipeline), int(persistent),
                       int(both_directions), 0.0))))
+
+    def set_short_divide(self, enable: bool) -> None:
+        self._c(load().tvs_batch_set_short_divide(self._h, int(enable)))
 
     def autotune(self, reps: int = 1):
         """Time every compiled launch shape on the whole batch, keep the
