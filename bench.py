@@ -37,9 +37,10 @@ HBM_PEAK_GBPS = 8000.0
 # VALU lane-operations per second: 256 CUs x 64 lanes x 2.4 GHz.
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 # Vector instructions per real band-step of the batched walk's unrolled
-# loop (sub, 4 for the divide, sub, cmpx, add, max; tests/test_codegen.py
-# holds the compiled loop to this count).
-VALU_PER_STEP = 9
+# loop: sub, the divide (4 operations, or 3 with the verified short divide),
+# sub, cmpx, add, max (tests/test_codegen.py holds the compiled loops to
+# these counts).
+VALU_PER_STEP = {"split": 9, "short": 8}
 
 
 def log(msg: str) -> None:
@@ -320,6 +321,7 @@ def main() -> int:
 
     log("timing batched (int16 heights, all sectors in one launch)")
     b_ms, b_kernel_ms, tm = time_method(e, "batched", args.steps, args.warmup)
+    _, short_sectors = e.short_divide()
     value = real_point_steps / (b_ms / 1000.0)
     log(f"batched: {b_ms:.2f} ms/step (walk kernel {b_kernel_ms:.2f} ms, reduce "
         f"{tm.reduce_ms:.2f} ms), {value:.3e} executed point-steps/s "
@@ -356,7 +358,8 @@ def main() -> int:
 
     bytes_per_launch = real_point_steps * 2     # one int16 height per executed step
     achieved = bytes_per_launch / (b_kernel_ms / 1000.0) / 1e9
-    valu_lane_ops = real_point_steps * VALU_PER_STEP / (b_kernel_ms / 1000.0)
+    per_step = VALU_PER_STEP["short"] if short_sectors == 180 else VALU_PER_STEP["split"]
+    valu_lane_ops = real_point_steps * per_step / (b_kernel_ms / 1000.0)
     traffic = None
     prof = os.path.join(ROOT, "profiles", "walk_tiles_main-1201_traffic.json")
     if os.path.exists(prof):
@@ -406,7 +409,8 @@ def main() -> int:
         },
         "nominal_value": point_steps / (b_ms / 1000.0),
         "valu": {
-            "instructions_per_step": VALU_PER_STEP,
+            "instructions_per_step": per_step,
+            "short_divide_sectors": short_sectors,
             "achieved_lane_ops_per_s": valu_lane_ops,
             "peak_lane_ops_per_s": VALU_PEAK_LANE_OPS,
             "frac": valu_lane_ops / VALU_PEAK_LANE_OPS,

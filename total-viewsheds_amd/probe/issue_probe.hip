@@ -1,8 +1,8 @@
 // issue_probe -- the instruction-issue ceiling of the batched walk's step.
 //
 // Runs the walk's per-step vector sequence (sub, mul, fmac, fma, fmac,
-// subrev, v_cmpx, masked add, exec restore, max: 9 VALU + 1 SALU) out of
-// registers, with no memory access, on every SIMD at full occupancy, and
+// subrev, v_cmpx, masked add, exec restore, max: 9 VALU + 1 SALU; 8 VALU
+// with the short divide) out of registers, with no memory access, on every SIMD at full occupancy, and
 // reports nanoseconds per wave-instruction per SIMD.  Next to it: the same
 // count of independent FMAs and of independent adds.  The walk kernel's own
 // rate (kernel time / its VALU instructions per SIMD) is compared with these
@@ -21,7 +21,8 @@ constexpr int kBlock = 256;
 
 #define TVS_REP8(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
 
-// mode 0: the walk step; 1: independent v_fma_f32; 2: independent v_add_f32.
+// mode 0: the walk step; 1: independent v_fma_f32; 2: independent v_add_f32;
+// 3: the walk step with the short divide.
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void issue_kernel(float *out, int iters, float c0,
                                                        float c1) {
@@ -46,11 +47,25 @@ __global__ __launch_bounds__(kBlock) void issue_kernel(float *out, int iters, fl
                  : "+v"(x##j), "+v"(y##j)                                              \
                  : "v"(h0), "s"(c0), "s"(c1), "s"(lanes)                               \
                  : "vcc");
+#define TVS_STEP3(j)                                                                   \
+    asm volatile("v_sub_f32_e32 %0, %0, %2\n\t"                                        \
+                 "v_mul_f32_e32 %1, %3, %0\n\t"                                        \
+                 "v_fma_f32 %0, -%1, %3, %0\n\t"                                       \
+                 "v_fmac_f32_e32 %1, %4, %0\n\t"                                       \
+                 "v_subrev_f32_e32 %1, %3, %1\n\t"                                     \
+                 "v_cmpx_gt_f32 vcc, %1, %2\n\t"                                       \
+                 "v_add_f32_e32 %0, %4, %0\n\t"                                        \
+                 "s_mov_b64 exec, %5\n\t"                                              \
+                 "v_max_f32_e32 %0, %1, %0"                                            \
+                 : "+v"(x##j), "+v"(y##j)                                              \
+                 : "v"(h0), "s"(c0), "s"(c1), "s"(lanes)                               \
+                 : "vcc");
 #define TVS_FMA(j) asm volatile("v_fma_f32 %0, %1, %2, %0" : "+v"(x##j) : "v"(y##j), "s"(c0));
 #define TVS_ADD(j) asm volatile("v_add_f32_e32 %0, %1, %0" : "+v"(x##j) : "s"(c1));
         if (MODE == 0) { TVS_REP8(TVS_STEP) }
         if (MODE == 1) { TVS_REP8(TVS_FMA) }
         if (MODE == 2) { TVS_REP8(TVS_ADD) }
+        if (MODE == 3) { TVS_REP8(TVS_STEP3) }
     }
     out[blockIdx.x * kBlock + threadIdx.x] =
         x0 + x1 + x2 + x3 + x4 + x5 + x6 + x7 + y0 + y1 + y2 + y3 + y4 + y5 + y6 + y7;
@@ -94,6 +109,7 @@ int main() {
     if (hipMalloc(&out, static_cast<size_t>(blocks) * kBlock * sizeof(float)) != hipSuccess)
         return 1;
     int rc = run<0>(out, blocks, 20000, simds, 8 * 9, "walk step (9 VALU + 1 SALU)");
+    rc |= run<3>(out, blocks, 20000, simds, 8 * 8, "walk step, short divide (8 VALU + 1 SALU)");
     rc |= run<1>(out, blocks, 20000, simds, 8, "independent v_fma_f32");
     rc |= run<2>(out, blocks, 20000, simds, 8, "independent v_add_f32");
     (void)hipFree(out);
