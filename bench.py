@@ -150,7 +150,8 @@ def measure_tiles(steps: int, warmup: int) -> dict:
         bands_ms = b.timing().bands_ms
         shape, _ = b.autotune(reps=1)
         log(f"tiles-64 tuned shape: block {shape['block']}, unroll {shape['unroll']}, "
-            f"xcd {shape['xcd_contiguous']}, both directions {shape['both_directions']}")
+            f"xcd {shape['xcd_contiguous']}, both directions {shape['both_directions']}, "
+            f"two points {shape['two_points']}")
         for _ in range(warmup):
             b.run(True)
         b.sync()
@@ -175,7 +176,8 @@ def measure_tiles(steps: int, warmup: int) -> dict:
             "tiles": len(tiles),
             "sides": sorted({t.width for t in tiles}),
             "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous",
-                                                 "pipeline", "persistent", "both_directions")}}
+                                                 "pipeline", "persistent", "both_directions",
+                                                 "two_points")}}
 
 
 def copy_calibration(mib: int) -> int:
@@ -250,10 +252,10 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true", help="skip the plain method")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default walk shape")
-    ap.add_argument("--shape", metavar="B,U,X,P,S,D",
+    ap.add_argument("--shape", metavar="B,U,X,P,S,D,T",
                     help="walk shape instead of tuning: points per item, unroll, "
-                         "XCD-contiguous, pipeline, persistent, both directions per lane "
-                         "(e.g. 64,8,1,0,0,0)")
+                         "XCD-contiguous, pipeline, persistent, both directions per lane, "
+                         "two points per lane (e.g. 64,8,1,0,0,0,0)")
     ap.add_argument("--no-f32", action="store_true", help="skip the f32-heights batched run")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the large-3601 and tiles-64 figures")
@@ -296,19 +298,21 @@ def main() -> int:
     real_point_steps = 180 * 2 * g.computable_points_count * g.real_steps
 
     if args.shape:
-        b, u, x, p, ps, dd = (int(v) for v in args.shape.split(","))
-        e.set_walk_shape(b, u, bool(x), bool(p), bool(ps), bool(dd))
+        b, u, x, p, ps, dd, tw = (int(v) for v in args.shape.split(","))
+        e.set_walk_shape(b, u, bool(x), bool(p), bool(ps), bool(dd), bool(tw))
     shape = e.walk_shape()
     if not args.no_tune and not args.shape:
         shape, table = e.autotune(0, 180, reps=1)
         log(f"autotuned walk shape: block {shape['block']}, unroll {shape['unroll']}, "
             f"xcd-contiguous {shape['xcd_contiguous']}, pipeline {shape['pipeline']}, "
-            f"persistent {shape['persistent']}, both directions {shape['both_directions']} "
+            f"persistent {shape['persistent']}, both directions {shape['both_directions']}, "
+            f"two points {shape['two_points']} "
             f"({shape['ms']:.2f} ms)")
         for row in sorted(table, key=lambda r: r["ms"]):
             log(f"  shape {row['block']:4d} x {row['unroll']:2d} xcd={int(row['xcd_contiguous'])} "
                 f"pipe={int(row['pipeline'])} pers={int(row['persistent'])} "
-                f"pair={int(row['both_directions'])}: {row['ms']:.3f} ms")
+                f"pair={int(row['both_directions'])} two={int(row['two_points'])}: "
+                f"{row['ms']:.3f} ms")
 
     plain = None
     if not args.no_plain:
@@ -335,7 +339,8 @@ def main() -> int:
             ef.upload(wl.elevations.astype(np.float32))
             ef.compute_bands(0, 180)
             ef.set_walk_shape(shape["block"], shape["unroll"], shape["xcd_contiguous"],
-                              shape["pipeline"], shape["persistent"], shape["both_directions"])
+                              shape["pipeline"], shape["persistent"], shape["both_directions"],
+                              shape["two_points"])
             f_ms, f_kernel_ms, _ = time_method(ef, "batched", args.steps, args.warmup)
         batched_f32 = {"value": real_point_steps / (f_ms / 1000.0), "ms_per_step": f_ms,
                        "nominal_value": point_steps / (f_ms / 1000.0),
@@ -394,7 +399,8 @@ def main() -> int:
             "steps_per_band": g.band_deltas_size,
             "real_steps_per_band": g.real_steps,
             "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous",
-                                                 "pipeline", "persistent", "both_directions")},
+                                                 "pipeline", "persistent", "both_directions",
+                                                 "two_points")},
         },
         "roofline": {
             "bound": "hbm",

@@ -261,7 +261,9 @@ const char *tvs_status_string(tvs_status status);
  * whether a grid of a few blocks per CU strides over all work items
  * (`persistent`, which ignores `xcd_contiguous`), and whether one lane walks
  * both bands (forward and backward) of its point, sharing the per-step
- * constants (`both_directions`); `ms` is a timing result.
+ * constants (`both_directions`), and whether one lane walks two adjacent
+ * points of a row, fetching both int16 heights with one load per step
+ * (`two_points`); `ms` is a timing result.
  * No reference counterpart: the Vulkan path's fixed 8x8x4 workgroup
  * (lib.rs:23, gpu.rs:10) is the only shape the reference has. */
 typedef struct tvs_walk_shape {
@@ -271,6 +273,7 @@ typedef struct tvs_walk_shape {
     uint32_t pipeline;
     uint32_t persistent;
     uint32_t both_directions;
+    uint32_t two_points;
     float ms;
 } tvs_walk_shape;
 

@@ -66,14 +66,17 @@ def test_batch_shapes_agree(tiles_and_truth):
         for i, t in enumerate(tiles):
             b.upload(i, t.elevations)
         b.compute_bands()
-        for block, unroll, xcd, pipe, pair in [(64, 8, False, False, False),
-                                               (128, 16, True, False, False),
-                                               (512, 8, False, True, False),
-                                               (256, 4, True, True, False),
-                                               (64, 8, False, False, True),
-                                               (128, 4, True, False, True)]:
-            print(f"batch shape {block}x{unroll} xcd={xcd} pipe={pipe} pair={pair}", flush=True)
-            b.set_walk_shape(block, unroll, xcd, pipe, both_directions=pair)
+        for block, unroll, xcd, pipe, pair, two in [(64, 8, False, False, False, False),
+                                                    (128, 16, True, False, False, False),
+                                                    (512, 8, False, True, False, False),
+                                                    (256, 4, True, True, False, False),
+                                                    (64, 8, False, False, True, False),
+                                                    (128, 4, True, False, True, False),
+                                                    (128, 8, False, False, False, True),
+                                                    (64, 4, True, False, True, True)]:
+            print(f"batch shape {block}x{unroll} xcd={xcd} pipe={pipe} pair={pair} two={two}",
+                  flush=True)
+            b.set_walk_shape(block, unroll, xcd, pipe, both_directions=pair, two_points=two)
             b.reset_surfaces()
             b.run(True)
             for i in range(len(tiles)):
@@ -91,7 +94,7 @@ def test_autotune_keeps_results_exact(tiles_and_truth):
         before = e.surfaces()
         best, table = e.autotune(0, 30, reps=1)
         print("autotune", best, flush=True)
-        assert len(table) == 2 * 16 + 3 and best["ms"] > 0   # persistent shapes once
+        assert len(table) == 2 * 22 + 3 and best["ms"] > 0   # persistent shapes once
         np.testing.assert_array_equal(e.surfaces(), before)   # surfaces preserved
         assert e.walk_shape()["block"] == best["block"]
         e.reset_surfaces()
@@ -112,7 +115,7 @@ def test_batch_autotune_keeps_results_exact(tiles_and_truth):
         before = [b.surfaces(i) for i in range(len(tiles))]
         best, table = b.autotune(reps=1)
         print("batch autotune", best, flush=True)
-        assert len(table) == 2 * 16 + 3 and best["ms"] > 0
+        assert len(table) == 2 * 22 + 3 and best["ms"] > 0
         for i in range(len(tiles)):
             np.testing.assert_array_equal(b.surfaces(i), before[i])
         b.reset_surfaces()

@@ -346,14 +346,20 @@ def test_ring_overflow_is_counted_not_spilled(small):
         assert np.all(r[:, 0] % 2 == 0)
 
 
-SHAPES = [  # (block, unroll, pipeline, persistent, both directions per lane)
-    (256, 8, False, False, False), (128, 8, False, False, False), (64, 8, False, False, False),
-    (512, 8, False, False, False), (256, 4, False, False, False), (256, 16, False, False, False),
-    (128, 16, False, False, False), (256, 8, True, False, False), (128, 8, True, False, False),
-    (256, 4, True, False, False), (512, 8, True, False, False), (256, 8, False, True, False),
-    (128, 8, False, True, False), (64, 8, False, True, False), (64, 8, False, False, True),
-    (128, 8, False, False, True), (256, 8, False, False, True), (64, 4, False, False, True),
-    (128, 4, False, False, True)]
+SHAPES = [  # (block, unroll, pipeline, persistent, both directions, two points per lane)
+    (256, 8, False, False, False, False), (128, 8, False, False, False, False),
+    (64, 8, False, False, False, False), (512, 8, False, False, False, False),
+    (256, 4, False, False, False, False), (256, 16, False, False, False, False),
+    (128, 16, False, False, False, False), (256, 8, True, False, False, False),
+    (128, 8, True, False, False, False), (256, 4, True, False, False, False),
+    (512, 8, True, False, False, False), (256, 8, False, True, False, False),
+    (128, 8, False, True, False, False), (64, 8, False, True, False, False),
+    (64, 8, False, False, True, False), (128, 8, False, False, True, False),
+    (256, 8, False, False, True, False), (64, 4, False, False, True, False),
+    (128, 4, False, False, True, False), (64, 8, False, False, False, True),
+    (128, 8, False, False, False, True), (64, 4, False, False, False, True),
+    (64, 4, False, False, True, True), (128, 4, False, False, True, True),
+    (64, 8, False, False, True, True)]
 
 
 @pytest.mark.parametrize("xcd", [True, False])
@@ -364,11 +370,12 @@ def test_every_walk_shape_bit_exact(small, xcd):
     with make_engine(wl) as e:
         e.upload(wl.elevations)
         e.compute_bands(0, 180)
-        for block, unroll, pipe, pers, pair in SHAPES:
-            note(f"shape {block}x{unroll} pipe={pipe} pers={pers} pair={pair} xcd={xcd}")
-            e.set_walk_shape(block, unroll, xcd, pipe, pers, pair)
+        for block, unroll, pipe, pers, pair, two in SHAPES:
+            note(f"shape {block}x{unroll} pipe={pipe} pers={pers} pair={pair} two={two} "
+                 f"xcd={xcd}")
+            e.set_walk_shape(block, unroll, xcd, pipe, pers, pair, two)
             e.reset_surfaces()
             e.run(0, 180, "batched")
             np.testing.assert_array_equal(e.surfaces(), cum,
                                           err_msg=f"shape {block}x{unroll} pipe={pipe} "
-                                                  f"pers={pers} pair={pair}")
+                                                  f"pers={pers} pair={pair} two={two}")

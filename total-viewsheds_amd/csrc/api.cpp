@@ -300,7 +300,7 @@ tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t cap
     d.L = c->geo.max_los_as_points;
     d.w = c->geo.dem_width;
     d.obs = c->params.observer_height;
-    d.blocks_per_pass = (N + cfg.block - 1) / cfg.block;
+    d.blocks_per_pass = walk_blocks_per_pass(c->geo.tvs_width, cfg);
     d.R = c->rings_per_band;
     d.capture_slots = capture;
     d.fast = c->fast_div_ok ? 1u : 0u;
@@ -695,8 +695,10 @@ tvs_status tvs_run_sectors(tvs_ctx *c, uint32_t first, uint32_t count, tvs_metho
         TVS_HIP(c, hipEventRecord(c->ev[2], c->stream), "event");
         c->timing.walk_launches = count;
     } else {
+        // Ring capture runs the capture shape; its items must be counted for it.
+        const tvs::WalkConfig cfg = capture ? tvs::capture_walk_config(c->walk_cfg) : c->walk_cfg;
         tvs::TileDesc d;
-        if ((st = tvs::prepare_tile(c, first, count, capture, c->walk_cfg, &d)) != TVS_OK)
+        if ((st = tvs::prepare_tile(c, first, count, capture, cfg, &d)) != TVS_OK)
             return st;
         if ((st = tvs::ensure_div3(c, c->tab, first, count)) != TVS_OK) return st;
         const uint32_t n_items = d.blocks_per_pass * 2u * count;
@@ -708,7 +710,7 @@ tvs_status tvs_run_sectors(tvs_ctx *c, uint32_t first, uint32_t count, tvs_metho
                 "tile descriptor");
         TVS_HIP(c, hipEventRecord(c->ev[0], c->stream), "event");
         TVS_HIP(c, tvs::launch_walk_tiles(c->stream, c->d_tile, c->d_begin, 1, n_items,
-                                          c->has_i16, capture > 0, c->walk_cfg),
+                                          c->has_i16, capture > 0, cfg),
                 "batched walk");
         TVS_HIP(c, hipEventRecord(c->ev[1], c->stream), "event");
         TVS_HIP(c, tvs::launch_reduce_tiles(c->stream, c->d_tile, c->d_begin + 2, 1, N),
@@ -765,7 +767,7 @@ tvs_status tvs_compact_rings(tvs_ctx *c, uint32_t sector, uint32_t *counts, size
         TVS_HIP(c, dev_alloc(&c->d_ring_counts, bands), "ring counts");
         TVS_HIP(c, dev_alloc(&c->d_ring_offsets, bands), "ring offsets");
     }
-    tvs::WalkConfig cfg;   // the capture path has one shape
+    const tvs::WalkConfig cfg = tvs::capture_walk_config(c->walk_cfg);
     tvs::TileDesc d;
     tvs_status st = tvs::prepare_tile(c, sector, 1, 1, cfg, &d);
     if (st != TVS_OK) return st;

@@ -255,7 +255,7 @@ tvs_status tvs_batch_set_walk_shape(tvs_batch *b, const tvs_walk_shape *shape) {
     if (!b || !shape) return bfail(b, TVS_ERR_INVALID_ARGUMENT, "null argument");
     tvs::WalkConfig cfg{shape->block, shape->unroll, shape->xcd_contiguous != 0,
                         shape->pipeline != 0, shape->persistent != 0,
-                        shape->both_directions != 0};
+                        shape->both_directions != 0, shape->two_points != 0};
     if (!tvs::walk_shape_supported(cfg))
         return bfail(b, TVS_ERR_INVALID_ARGUMENT, "walk shape not compiled");
     b->cfg = cfg;
@@ -300,7 +300,7 @@ tvs_status tvs_batch_autotune(tvs_batch *b, uint32_t reps, tvs_walk_shape *resul
         if (results && k < max_results)
             results[k] = tvs_walk_shape{cfg.block, cfg.unroll, cfg.xcd_contiguous ? 1u : 0u,
                                         cfg.pipeline ? 1u : 0u, cfg.persistent ? 1u : 0u,
-                                        cfg.pair ? 1u : 0u, ms};
+                                        cfg.pair ? 1u : 0u, cfg.twin ? 1u : 0u, ms};
         ++k;
         if (ms < winner_ms) {
             winner_ms = ms;
@@ -315,7 +315,7 @@ tvs_status tvs_batch_autotune(tvs_batch *b, uint32_t reps, tvs_walk_shape *resul
     if (best)
         *best = tvs_walk_shape{winner.block, winner.unroll, winner.xcd_contiguous ? 1u : 0u,
                                winner.pipeline ? 1u : 0u, winner.persistent ? 1u : 0u,
-                               winner.pair ? 1u : 0u, winner_ms};
+                               winner.pair ? 1u : 0u, winner.twin ? 1u : 0u, winner_ms};
     return TVS_OK;
 }
 

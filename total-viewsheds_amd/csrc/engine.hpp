@@ -105,20 +105,41 @@ struct WalkConfig {
     bool pipeline = false;    // next chunk's heights requested before this chunk's math
     bool persistent = false;  // a few blocks per CU stride over all items
     bool pair = false;        // one lane walks both bands (forward, backward) of a point
+    bool twin = false;        // one lane walks two adjacent points of a row
 };
 
-// Compiled (block, unroll, pipeline, persistent, pair) shapes of the tile walk.
+// The one shape of ring-capturing walks (a checking path): 256 points per
+// block, 8 steps per chunk, one band per lane.
+inline WalkConfig capture_walk_config(const WalkConfig &cfg) {
+    return WalkConfig{256, 8, cfg.xcd_contiguous, false, false, false, false};
+}
+
+// Blocks per pass (per sector and direction, or per sector with `pair`) of
+// the tile walk for N = T*T points: BLOCK points per block, or with `twin`
+// the T*(T/2) column pairs two per lane plus (odd T) the T last-column
+// points one per lane.
+inline uint32_t walk_blocks_per_pass(uint32_t T, const WalkConfig &cfg) {
+    if (!cfg.twin) return (T * T + cfg.block - 1) / cfg.block;
+    const uint32_t pairs = T * (T / 2u);
+    return (pairs + cfg.block - 1) / cfg.block + ((T & 1u) ? (T + cfg.block - 1) / cfg.block : 0u);
+}
+
+// Compiled (block, unroll, pipeline, persistent, pair, twin) shapes of the
+// tile walk.
 #define TVS_WALK_SHAPES(X)                                                               \
-    X(256, 8, false, false, false) X(128, 8, false, false, false)                        \
-    X(64, 8, false, false, false) X(512, 8, false, false, false)                         \
-    X(256, 4, false, false, false) X(256, 16, false, false, false)                       \
-    X(128, 16, false, false, false) X(256, 8, true, false, false)                        \
-    X(128, 8, true, false, false) X(256, 4, true, false, false)                          \
-    X(512, 8, true, false, false) X(256, 8, false, true, false)                          \
-    X(128, 8, false, true, false) X(64, 8, false, true, false)                           \
-    X(64, 8, false, false, true) X(128, 8, false, false, true)                           \
-    X(256, 8, false, false, true) X(64, 4, false, false, true)                           \
-    X(128, 4, false, false, true)
+    X(256, 8, false, false, false, false) X(128, 8, false, false, false, false)          \
+    X(64, 8, false, false, false, false) X(512, 8, false, false, false, false)           \
+    X(256, 4, false, false, false, false) X(256, 16, false, false, false, false)         \
+    X(128, 16, false, false, false, false) X(256, 8, true, false, false, false)          \
+    X(128, 8, true, false, false, false) X(256, 4, true, false, false, false)            \
+    X(512, 8, true, false, false, false) X(256, 8, false, true, false, false)            \
+    X(128, 8, false, true, false, false) X(64, 8, false, true, false, false)             \
+    X(64, 8, false, false, true, false) X(128, 8, false, false, true, false)             \
+    X(256, 8, false, false, true, false) X(64, 4, false, false, true, false)             \
+    X(128, 4, false, false, true, false) X(64, 8, false, false, false, true)             \
+    X(128, 8, false, false, false, true) X(64, 4, false, false, false, true)             \
+    X(64, 4, false, false, true, true) X(128, 4, false, false, true, true)               \
+    X(64, 8, false, false, true, true)
 
 // Scratch for the device Axes::compute (keys, ids, radix-sort ping-pong,
 // histograms).

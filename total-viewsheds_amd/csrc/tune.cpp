@@ -3,7 +3,7 @@
 // The shapes compiled into the library (points per work item = threads per
 // block, steps unrolled per chunk, software pipelining, XCD-contiguous or
 // round-robin item mapping, a persistent grid, both bands of a point per
-// lane) are timed on the context's own DEM and band data; the fastest
+// lane, two points per lane) are timed on the context's own DEM and band data; the fastest
 // becomes the context's shape.  The cumulative surfaces are saved and
 // restored around the timing runs.
 #include <algorithm>
@@ -17,9 +17,9 @@ namespace tvs {
 std::vector<WalkConfig> compiled_walk_shapes() {
     std::vector<WalkConfig> out;
     // A persistent grid strides over the items itself: one item mapping.
-#define TVS_SHAPE(B, U, P, S, D)                                   \
-    if (!S) out.push_back(WalkConfig{B, U, true, P, S, D});   \
-    out.push_back(WalkConfig{B, U, false, P, S, D});
+#define TVS_SHAPE(B, U, P, S, D, W)                                   \
+    if (!S) out.push_back(WalkConfig{B, U, true, P, S, D, W});   \
+    out.push_back(WalkConfig{B, U, false, P, S, D, W});
     TVS_WALK_SHAPES(TVS_SHAPE)
 #undef TVS_SHAPE
     return out;
@@ -42,7 +42,7 @@ tvs_status tvs_set_walk_shape(tvs_ctx *c, const tvs_walk_shape *shape) {
     if (!c || !shape) return TVS_ERR_INVALID_ARGUMENT;
     tvs::WalkConfig cfg{shape->block, shape->unroll, shape->xcd_contiguous != 0,
                         shape->pipeline != 0, shape->persistent != 0,
-                        shape->both_directions != 0};
+                        shape->both_directions != 0, shape->two_points != 0};
     if (!tvs::walk_shape_supported(cfg))
         return tfail(c, TVS_ERR_INVALID_ARGUMENT, "walk shape not compiled");
     c->walk_cfg = cfg;
@@ -57,6 +57,7 @@ tvs_status tvs_get_walk_shape(const tvs_ctx *c, tvs_walk_shape *out) {
     out->pipeline = c->walk_cfg.pipeline ? 1u : 0u;
     out->persistent = c->walk_cfg.persistent ? 1u : 0u;
     out->both_directions = c->walk_cfg.pair ? 1u : 0u;
+    out->two_points = c->walk_cfg.twin ? 1u : 0u;
     out->ms = 0.0f;
     return TVS_OK;
 }
@@ -104,7 +105,7 @@ tvs_status tvs_autotune(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t rep
         if (results && k < max_results)
             results[k] = tvs_walk_shape{cfg.block, cfg.unroll, cfg.xcd_contiguous ? 1u : 0u,
                                         cfg.pipeline ? 1u : 0u, cfg.persistent ? 1u : 0u,
-                                        cfg.pair ? 1u : 0u, ms};
+                                        cfg.pair ? 1u : 0u, cfg.twin ? 1u : 0u, ms};
         ++k;
         if (ms < winner_ms) {
             winner_ms = ms;
@@ -124,7 +125,7 @@ tvs_status tvs_autotune(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t rep
     if (best)
         *best = tvs_walk_shape{winner.block, winner.unroll, winner.xcd_contiguous ? 1u : 0u,
                                winner.pipeline ? 1u : 0u, winner.persistent ? 1u : 0u,
-                               winner.pair ? 1u : 0u, winner_ms};
+                               winner.pair ? 1u : 0u, winner.twin ? 1u : 0u, winner_ms};
     return TVS_OK;
 }
 

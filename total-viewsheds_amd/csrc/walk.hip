@@ -427,6 +427,109 @@ __device__ __forceinline__ void walk_pair(const TileDesc &d, uint32_t rel, uint3
     }
 }
 
+// Heights of two adjacent elements (the step's position for a lane's point
+// t0 and for t0 + 1, which sit next to each other in the row).  int16: one
+// dword load returns both, so a band-step costs half a load instruction
+// (the texture unit's rate per load instruction bounds this walk); f32:
+// two loads.  `raw` is an untyped resource over the same range as the
+// heights' typed one.
+template <typename H>
+__device__ __forceinline__ void load_pair(__amdgpu_buffer_rsrc_t raw, uint32_t pov_bytes,
+                                          uint32_t pos16, float &a, float &b);
+
+template <>
+__device__ __forceinline__ void load_pair<int16_t>(__amdgpu_buffer_rsrc_t raw, uint32_t pov_bytes,
+                                                   uint32_t pos16, float &a, float &b) {
+    const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(raw, pov_bytes, pos16, 0);
+    a = static_cast<float>(static_cast<int16_t>(w & 0xffffu));
+    b = static_cast<float>(static_cast<int32_t>(w) >> 16);
+}
+
+template <>
+__device__ __forceinline__ void load_pair<float>(__amdgpu_buffer_rsrc_t raw, uint32_t pov_bytes,
+                                                 uint32_t pos16, float &a, float &b) {
+    a = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(raw, pov_bytes, pos16 << 1, 0));
+    b = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(raw, pov_bytes + 4u, pos16 << 1, 0));
+}
+
+// Two adjacent points of a row per lane, t0 and t0 + 1, in one direction
+// or (BOTH) in both, no ring capture.  Bands per lane: point t0 and t0 + 1
+// forward-or-`backward`, then (BOTH) t0 and t0 + 1 backward.  Every band
+// keeps its own max and surface; the per-step constants and table reads
+// are shared by all of them.  The zero tail is skipped as in walk_pair.
+template <typename H, int DIV, int UNROLL, bool BOTH>
+__device__ __forceinline__ void walk_twin(const TileDesc &d, uint32_t rel, bool backward,
+                                          uint32_t t0, bool active) {
+    constexpr int NB = BOTH ? 4 : 2;
+    const H *elev = static_cast<const H *>(d.elev);
+    const uint32_t slot = d.first_slot + rel;
+    const uint32_t pov = (t0 / d.T + d.L) * d.w + (t0 % d.T + d.L);
+    const uint32_t pov_bytes = pov * static_cast<uint32_t>(sizeof(H));
+    const float h0a = static_cast<float>(global_ptr(elev)[pov]) + d.obs;
+    const float h0b = static_cast<float>(global_ptr(elev)[pov + 1u]) + d.obs;
+    const uint32_t n_fast = global_ptr(d.meta)[slot].n_fast;
+    const uint32_t tail_start = global_ptr(d.meta)[slot].tail_start;
+    const uint32_t bias = global_ptr(d.meta)[slot].bias;
+    const __amdgpu_buffer_rsrc_t raw = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<H *>(elev - bias), 0,
+        static_cast<int>((bias + d.w * d.w) * sizeof(H)), kHeightRsrcFlags<float>);
+
+    const size_t tb = static_cast<size_t>(slot) * d.stride;
+    const TVS_GLOBAL uint32_t *__restrict__ pos0 =
+        global_ptr(((BOTH || !backward) ? d.pos_f : d.pos_b) + tb);
+    const TVS_GLOBAL uint32_t *__restrict__ pos1 = global_ptr(d.pos_b + tb);
+    const TVS_GLOBAL float *__restrict__ dist = global_ptr(d.dist + tb);
+    const TVS_GLOBAL float *__restrict__ rcp = global_ptr(d.rcp + tb);
+    const TVS_GLOBAL float *__restrict__ rcp_lo = global_ptr(d.rcp_lo + tb);
+    const TVS_GLOBAL float *__restrict__ curv = global_ptr(d.curv + tb);
+    const TVS_GLOBAL float *__restrict__ wt = global_ptr(d.wt + tb);
+    const uint64_t lanes = __builtin_amdgcn_read_exec();
+
+    float max_angle[NB], surface[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        max_angle[b] = kMaxAngle;
+        surface[b] = 0.0f;
+    }
+    auto step = [&](const float (&h)[NB], uint32_t k, bool fast) {
+        const float dk = dist[k];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const float dh = h[b] - ((b & 1) ? h0b : h0a);
+            const float q = fast ? step_div<DIV>(dh, dk, rcp[k], rcp_lo[k]) : dh / dk;
+            step_update(q - curv[k], wt[k], lanes, max_angle[b], surface[b]);
+        }
+    };
+    uint32_t k = 0;
+    const uint32_t fast_end = n_fast - n_fast % UNROLL;
+    for (; k < fast_end; k += UNROLL) {
+        float h[UNROLL][NB];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            load_pair<H>(raw, pov_bytes, pos0[k + u], h[u][0], h[u][1]);
+            if (BOTH) load_pair<H>(raw, pov_bytes, pos1[k + u], h[u][NB - 2], h[u][NB - 1]);
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) step(h[u], k + u, DIV != 0);
+    }
+    for (; k < tail_start; ++k) {
+        float h[NB];
+        load_pair<H>(raw, pov_bytes, pos0[k], h[0], h[1]);
+        if (BOTH) load_pair<H>(raw, pov_bytes, pos1[k], h[NB - 2], h[NB - 1]);
+        step(h, k, DIV != 0 && k < n_fast);
+    }
+    if (active) {
+        TVS_GLOBAL float *out =
+            global_ptr(d.partial) + static_cast<size_t>(2u * rel + (BOTH ? 0u : (backward ? 1u : 0u))) * d.N + t0;
+        out[0] = surface[0];
+        out[1] = surface[1];
+        if (BOTH) {
+            out[d.N] = surface[NB - 2];
+            out[d.N + 1] = surface[NB - 1];
+        }
+    }
+}
+
 // Index of the tile whose [begin[i], begin[i+1]) holds x (wave-uniform).
 __device__ __forceinline__ uint32_t find_tile(const uint32_t *__restrict__ begin,
                                               uint32_t n_tiles, uint32_t x) {
@@ -450,22 +553,49 @@ __device__ __forceinline__ int divide_of(const TileDesc &d, uint32_t rel) {
 // A work item is (tile, sector, direction, BLOCK points).  With PAIR an item
 // is (tile, sector, BLOCK points), both directions: `item` counts pairs and
 // 2*item is the first of the pair's two single-direction items (every tile
-// holds an even number of them).
-template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PAIR>
+// holds an even number of them).  With TWIN the points are taken two per
+// lane: the first twin_blocks items of a pass hold the column pairs (2c,
+// 2c + 1) of every row (T/2 pairs per row), the rest (odd T only) the last
+// column of each row, one point per lane.
+template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PAIR, bool TWIN>
 __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
                                           const uint32_t *__restrict__ item_begin,
                                           uint32_t n_tiles, uint32_t item) {
+    const uint32_t x = PAIR ? 2u * item : item;
+    const uint32_t ti = n_tiles > 1 ? find_tile(item_begin, n_tiles, x) : 0u;
+    const TileDesc d = tiles[ti];
+    const uint32_t local = PAIR ? (x - item_begin[ti]) >> 1 : x - item_begin[ti];
+    const uint32_t pass = local / d.blocks_per_pass;     // sector (PAIR) or 2*sector + dir
+    const uint32_t ob = local - pass * d.blocks_per_pass;
+    const uint32_t rel = PAIR ? pass : pass >> 1;
+    const bool backward = !PAIR && (pass & 1u) != 0;
+    const int div = divide_of(d, rel);
+    if (TWIN) {
+        const uint32_t half = d.T >> 1;
+        const uint32_t n_pairs = d.T * half;
+        const uint32_t twin_blocks = (n_pairs + BLOCK - 1) / BLOCK;
+        if (ob < twin_blocks) {
+            const uint32_t p_raw = ob * BLOCK + threadIdx.x;
+            const bool active = p_raw < n_pairs;
+            const uint32_t p = active ? p_raw : n_pairs - 1u;
+            const uint32_t t0 = (p / half) * d.T + 2u * (p % half);
+            if (div == 3)
+                walk_twin<H, 3, UNROLL, PAIR>(d, rel, backward, t0, active);
+            else if (div == 4)
+                walk_twin<H, 4, UNROLL, PAIR>(d, rel, backward, t0, active);
+            else
+                walk_twin<H, 0, UNROLL, PAIR>(d, rel, backward, t0, active);
+            return;
+        }
+    }
+    // One point per lane: consecutive points, or (TWIN) the last column.
+    const uint32_t lb = TWIN ? ob - (d.T * (d.T >> 1) + BLOCK - 1) / BLOCK : ob;
+    const uint32_t i_raw = lb * BLOCK + threadIdx.x;
+    const uint32_t n_single = TWIN ? d.T : d.N;
+    const bool active = i_raw < n_single;
+    const uint32_t i = active ? i_raw : n_single - 1u;   // spare lanes shadow
+    const uint32_t t = TWIN ? i * d.T + (d.T - 1u) : i;
     if (PAIR) {
-        const uint32_t x = 2u * item;
-        const uint32_t ti = n_tiles > 1 ? find_tile(item_begin, n_tiles, x) : 0u;
-        const TileDesc d = tiles[ti];
-        const uint32_t local = (x - item_begin[ti]) >> 1;
-        const uint32_t rel = local / d.blocks_per_pass;
-        const uint32_t ob = local - rel * d.blocks_per_pass;
-        const uint32_t t_raw = ob * BLOCK + threadIdx.x;
-        const bool active = t_raw < d.N;
-        const uint32_t t = active ? t_raw : d.N - 1u;
-        const int div = divide_of(d, rel);
         if (div == 3)
             walk_pair<H, 3, UNROLL>(d, rel, t, active);
         else if (div == 4)
@@ -474,17 +604,6 @@ __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
             walk_pair<H, 0, UNROLL>(d, rel, t, active);
         return;
     }
-    const uint32_t ti = n_tiles > 1 ? find_tile(item_begin, n_tiles, item) : 0u;
-    const TileDesc d = tiles[ti];
-    const uint32_t local = item - item_begin[ti];
-    const uint32_t pass = local / d.blocks_per_pass;     // 2*sector + dir
-    const uint32_t ob = local - pass * d.blocks_per_pass;
-    const uint32_t t_raw = ob * BLOCK + threadIdx.x;
-    const bool active = t_raw < d.N;
-    const uint32_t t = active ? t_raw : d.N - 1u;       // spare lanes shadow
-    const uint32_t rel = pass >> 1;
-    const bool backward = (pass & 1u) != 0;
-    const int div = divide_of(d, rel);
     if (div == 3)
         walk_band<H, 3, CAPTURE, UNROLL, PIPE>(d, rel, backward, t, active);
     else if (div == 4)
@@ -497,19 +616,21 @@ __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
 // striding over all items.  With items_per_xcd != 0, consecutive block ids
 // (dispatched to the 8 XCDs round-robin) are remapped so each XCD takes a
 // contiguous run of items: one DEM region per L2.
-template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PERSIST, bool PAIR>
+template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PERSIST, bool PAIR,
+          bool TWIN>
 __global__ __launch_bounds__(BLOCK) void walk_tiles_kernel(
     const TileDesc *__restrict__ tiles, const uint32_t *__restrict__ item_begin,
     uint32_t n_tiles, uint32_t n_items, uint32_t items_per_xcd) {
     const uint32_t b = blockIdx.x;
     if (PERSIST) {
         for (uint32_t item = b; item < n_items; item += gridDim.x)
-            walk_item<H, CAPTURE, BLOCK, UNROLL, PIPE, PAIR>(tiles, item_begin, n_tiles, item);
+            walk_item<H, CAPTURE, BLOCK, UNROLL, PIPE, PAIR, TWIN>(tiles, item_begin, n_tiles,
+                                                                   item);
         return;
     }
     const uint32_t item = items_per_xcd ? (b & 7u) * items_per_xcd + (b >> 3) : b;
     if (item >= n_items) return;
-    walk_item<H, CAPTURE, BLOCK, UNROLL, PIPE, PAIR>(tiles, item_begin, n_tiles, item);
+    walk_item<H, CAPTURE, BLOCK, UNROLL, PIPE, PAIR, TWIN>(tiles, item_begin, n_tiles, item);
 }
 
 // Per point, in sector order: cum += forward; cum += backward (the f32
@@ -603,7 +724,8 @@ uint32_t persistent_blocks(uint32_t block) {
 
 // `n_items` counts single-direction items; a PAIR launch has half as many
 // (every tile's count is even).
-template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PERSIST, bool PAIR>
+template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PERSIST, bool PAIR,
+          bool TWIN>
 hipError_t launch_tiles_t(hipStream_t s, const TileDesc *tiles, const uint32_t *item_begin,
                           uint32_t n_tiles, uint32_t n_items, bool xcd_contiguous) {
     if (PAIR && (n_items & 1u)) return hipErrorInvalidValue;
@@ -611,7 +733,7 @@ hipError_t launch_tiles_t(hipStream_t s, const TileDesc *tiles, const uint32_t *
     const uint32_t per_xcd = (xcd_contiguous && !PERSIST) ? (items + 7u) / 8u : 0u;
     uint32_t grid = per_xcd ? per_xcd * 8u : items;
     if (PERSIST) grid = std::min(items, persistent_blocks(BLOCK));
-    hipLaunchKernelGGL((walk_tiles_kernel<H, CAPTURE, BLOCK, UNROLL, PIPE, PERSIST, PAIR>),
+    hipLaunchKernelGGL((walk_tiles_kernel<H, CAPTURE, BLOCK, UNROLL, PIPE, PERSIST, PAIR, TWIN>),
                        dim3(grid), dim3(BLOCK), 0, s, tiles, item_begin, n_tiles, items, per_xcd);
     return hipGetLastError();
 }
@@ -620,14 +742,14 @@ template <typename H>
 hipError_t launch_tiles_h(hipStream_t s, const TileDesc *tiles, const uint32_t *item_begin,
                           uint32_t n_tiles, uint32_t n_items, bool capture,
                           const WalkConfig &cfg) {
-    if (capture)   // ring capture is a checking path: one shape
-        return launch_tiles_t<H, true, 256, 8, false, false, false>(
+    if (capture)   // ring capture is a checking path: one shape (capture_walk_config)
+        return launch_tiles_t<H, true, 256, 8, false, false, false, false>(
             s, tiles, item_begin, n_tiles, n_items, cfg.xcd_contiguous);
-#define TVS_SHAPE(B, U, P, S, D)                                                           \
+#define TVS_SHAPE(B, U, P, S, D, W)                                                        \
     if (cfg.block == B && cfg.unroll == U && cfg.pipeline == P && cfg.persistent == S &&   \
-        cfg.pair == D)                                                                     \
-        return launch_tiles_t<H, false, B, U, P, S, D>(s, tiles, item_begin, n_tiles,      \
-                                                       n_items, cfg.xcd_contiguous);
+        cfg.pair == D && cfg.twin == W)                                                    \
+        return launch_tiles_t<H, false, B, U, P, S, D, W>(s, tiles, item_begin, n_tiles,   \
+                                                          n_items, cfg.xcd_contiguous);
     TVS_WALK_SHAPES(TVS_SHAPE)
 #undef TVS_SHAPE
     return hipErrorInvalidValue;
@@ -649,9 +771,9 @@ hipError_t launch_walk_plain(tvs_ctx *c, uint32_t slot, uint32_t *ring_out) {
 }
 
 bool walk_shape_supported(const WalkConfig &cfg) {
-#define TVS_SHAPE(B, U, P, S, D)                                                         \
+#define TVS_SHAPE(B, U, P, S, D, W)                                                      \
     if (cfg.block == B && cfg.unroll == U && cfg.pipeline == P && cfg.persistent == S &&   \
-        cfg.pair == D)                                                                     \
+        cfg.pair == D && cfg.twin == W)                                                    \
         return true;
     TVS_WALK_SHAPES(TVS_SHAPE)
 #undef TVS_SHAPE

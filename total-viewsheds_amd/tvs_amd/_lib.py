@@ -101,6 +101,7 @@ class WalkShape(ctypes.Structure):
         ("pipeline", ctypes.c_uint32),
         ("persistent", ctypes.c_uint32),
         ("both_directions", ctypes.c_uint32),
+        ("two_points", ctypes.c_uint32),
         ("ms", ctypes.c_float),
     ]
 
@@ -108,7 +109,8 @@ class WalkShape(ctypes.Structure):
         return {"block": self.block, "unroll": self.unroll,
                 "xcd_contiguous": bool(self.xcd_contiguous),
                 "pipeline": bool(self.pipeline), "persistent": bool(self.persistent),
-                "both_directions": bool(self.both_directions), "ms": self.ms}
+                "both_directions": bool(self.both_directions),
+                "two_points": bool(self.two_points), "ms": self.ms}
 
 
 class TvsError(RuntimeError):
@@ -384,10 +386,10 @@ class Engine:
 
     def set_walk_shape(self, block: int, unroll: int, xcd_contiguous: bool = True,
                        pipeline: bool = False, persistent: bool = False,
-                       both_directions: bool = False) -> None:
+                       both_directions: bool = False, two_points: bool = False) -> None:
         self._c(load().tvs_set_walk_shape(self._h, ctypes.byref(
             WalkShape(block, unroll, int(xcd_contiguous), int(pipeline), int(persistent),
-                      int(both_directions), 0.0))))
+                      int(both_directions), int(two_points), 0.0))))
 
     def walk_shape(self) -> dict:
         w = WalkShape()
@@ -483,10 +485,10 @@ class Batch:
 
     def set_walk_shape(self, block: int, unroll: int, xcd_contiguous: bool = False,
                        pipeline: bool = False, persistent: bool = False,
-                       both_directions: bool = False) -> None:
+                       both_directions: bool = False, two_points: bool = False) -> None:
         self._c(load().tvs_batch_set_walk_shape(self._h, ctypes.byref(
             WalkShape(block, unroll, int(xcd_contiguous), int(pipeline), int(persistent),
-                      int(both_directions), 0.0))))
+                      int(both_directions), int(two_points), 0.0))))
 
     def set_short_divide(self, enable: bool) -> None:
         self._c(load().tvs_batch_set_short_divide(self._h, int(enable)))
