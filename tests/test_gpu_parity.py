@@ -198,6 +198,31 @@ def test_small_rings_bit_exact(small, oracle, method):
         assert e.ring_overflow() == 0
 
 
+@pytest.mark.parametrize("shape", [(512, 8, False, False), (64, 8, True, False),
+                                   (64, 4, True, True)])
+def test_rings_with_any_walk_shape(small, oracle, shape):
+    """Ring capture runs the capture shape whatever shape the context is set
+    to (a larger, paired or two-point shape once counted its items wrongly):
+    rings and surfaces match the oracle."""
+    wl, dem, deltas, dists, _ = small
+    first, count = 40, 3
+    block, unroll, pair, two = shape
+    note(f"small-201 rings with walk shape {shape}")
+    R = oracle.ring_count_per_band(wl.rings_per_km, wl.max_line_of_sight)
+    want_cum, want = oracle.walk_sectors(dem, wl.elevations.astype(np.float32),
+                                         wl.observer_height, deltas[first:first + count],
+                                         dists[first:first + count], rings_per_band=R,
+                                         rings=True)
+    with make_engine(wl, capture=count) as e:
+        e.upload(wl.elevations)
+        e.compute_bands(first, count)
+        e.set_walk_shape(block, unroll, True, both_directions=pair, two_points=two)
+        e.run(first, count, "batched")
+        for r in range(count):
+            np.testing.assert_array_equal(e.rings(r), want[r], err_msg=f"slot {r}")
+        np.testing.assert_array_equal(e.surfaces(), want_cum)
+
+
 # --- 1201 x 1201 -------------------------------------------------------------
 
 @pytest.fixture(scope="module")
