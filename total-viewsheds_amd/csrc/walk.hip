@@ -197,6 +197,18 @@ __device__ __forceinline__ TVS_GLOBAL T *global_ptr(T *p) {
     return (TVS_GLOBAL T *)p;
 }
 
+// The step tables and sector metadata are not written while a walk runs;
+// reading them through the constant address space lets every wave-uniform
+// read be a scalar load whatever stores the kernel makes elsewhere (with
+// global pointers the compiler must prove no store can alias the table,
+// and in the larger kernels it stops proving it and falls back to per-lane
+// vector loads, which compete with the height loads).
+#define TVS_CONST __attribute__((address_space(4)))
+template <typename T>
+__device__ __forceinline__ const TVS_CONST T *const_ptr(const T *p) {
+    return (const TVS_CONST T *)p;
+}
+
 // The divide of a regular step.  DIV 4: the split-reciprocal divide
 // (proven correctly rounded in the fast-divide range); DIV 3: the short
 // divide (used only for sectors verified against DIV 4); DIV 0: IEEE.
@@ -217,9 +229,9 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
     const uint32_t pov = (t / d.T + d.L) * d.w + (t % d.T + d.L);
     const uint32_t pov_bytes = pov * static_cast<uint32_t>(sizeof(H));
     const float h0 = static_cast<float>(global_ptr(elev)[pov]) + d.obs;
-    const uint32_t n_fast = global_ptr(d.meta)[slot].n_fast;
-    const uint32_t tail_start = global_ptr(d.meta)[slot].tail_start;
-    const uint32_t bias = global_ptr(d.meta)[slot].bias;
+    const uint32_t n_fast = const_ptr(d.meta)[slot].n_fast;
+    const uint32_t tail_start = const_ptr(d.meta)[slot].tail_start;
+    const uint32_t bias = const_ptr(d.meta)[slot].bias;
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<H *>(elev - bias), 0,
         static_cast<int>((bias + d.w * d.w) * sizeof(H)), kHeightRsrcFlags<H>);
@@ -227,13 +239,13 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
     const size_t tb = static_cast<size_t>(slot) * d.stride;
     // Position of each step relative to the resource base in int16 bytes:
     // 2 (bias + off) forward, 2 (bias - off) backward (from the step table).
-    const TVS_GLOBAL uint32_t *__restrict__ pos = global_ptr((backward ? d.pos_b : d.pos_f) + tb);
+    const TVS_CONST uint32_t *__restrict__ pos = const_ptr((backward ? d.pos_b : d.pos_f) + tb);
     const uint64_t lanes = __builtin_amdgcn_read_exec();
-    const TVS_GLOBAL float *__restrict__ dist = global_ptr(d.dist + tb);
-    const TVS_GLOBAL float *__restrict__ rcp = global_ptr(d.rcp + tb);
-    const TVS_GLOBAL float *__restrict__ rcp_lo = global_ptr(d.rcp_lo + tb);
-    const TVS_GLOBAL float *__restrict__ curv = global_ptr(d.curv + tb);
-    const TVS_GLOBAL float *__restrict__ wt = global_ptr(d.wt + tb);
+    const TVS_CONST float *__restrict__ dist = const_ptr(d.dist + tb);
+    const TVS_CONST float *__restrict__ rcp = const_ptr(d.rcp + tb);
+    const TVS_CONST float *__restrict__ rcp_lo = const_ptr(d.rcp_lo + tb);
+    const TVS_CONST float *__restrict__ curv = const_ptr(d.curv + tb);
+    const TVS_CONST float *__restrict__ wt = const_ptr(d.wt + tb);
 
     // `max_angle = v ? angle : max_angle` is f32::max(angle, max_angle)
     // (kernel.rs:233) for every input the walk meets: v is `angle >
@@ -372,22 +384,22 @@ __device__ __forceinline__ void walk_pair(const TileDesc &d, uint32_t rel, uint3
     const uint32_t pov = (t / d.T + d.L) * d.w + (t % d.T + d.L);
     const uint32_t pov_bytes = pov * static_cast<uint32_t>(sizeof(H));
     const float h0 = static_cast<float>(global_ptr(elev)[pov]) + d.obs;
-    const uint32_t n_fast = global_ptr(d.meta)[slot].n_fast;
-    const uint32_t tail_start = global_ptr(d.meta)[slot].tail_start;
-    const uint32_t bias = global_ptr(d.meta)[slot].bias;
+    const uint32_t n_fast = const_ptr(d.meta)[slot].n_fast;
+    const uint32_t tail_start = const_ptr(d.meta)[slot].tail_start;
+    const uint32_t bias = const_ptr(d.meta)[slot].bias;
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<H *>(elev - bias), 0,
         static_cast<int>((bias + d.w * d.w) * sizeof(H)), kHeightRsrcFlags<H>);
 
     const size_t tb = static_cast<size_t>(slot) * d.stride;
-    const TVS_GLOBAL uint32_t *__restrict__ pos_f = global_ptr(d.pos_f + tb);
-    const TVS_GLOBAL uint32_t *__restrict__ pos_b = global_ptr(d.pos_b + tb);
+    const TVS_CONST uint32_t *__restrict__ pos_f = const_ptr(d.pos_f + tb);
+    const TVS_CONST uint32_t *__restrict__ pos_b = const_ptr(d.pos_b + tb);
     const uint64_t lanes = __builtin_amdgcn_read_exec();
-    const TVS_GLOBAL float *__restrict__ dist = global_ptr(d.dist + tb);
-    const TVS_GLOBAL float *__restrict__ rcp = global_ptr(d.rcp + tb);
-    const TVS_GLOBAL float *__restrict__ rcp_lo = global_ptr(d.rcp_lo + tb);
-    const TVS_GLOBAL float *__restrict__ curv = global_ptr(d.curv + tb);
-    const TVS_GLOBAL float *__restrict__ wt = global_ptr(d.wt + tb);
+    const TVS_CONST float *__restrict__ dist = const_ptr(d.dist + tb);
+    const TVS_CONST float *__restrict__ rcp = const_ptr(d.rcp + tb);
+    const TVS_CONST float *__restrict__ rcp_lo = const_ptr(d.rcp_lo + tb);
+    const TVS_CONST float *__restrict__ curv = const_ptr(d.curv + tb);
+    const TVS_CONST float *__restrict__ wt = const_ptr(d.wt + tb);
 
     float max_f = kMaxAngle, max_b = kMaxAngle;
     float surface_f = 0.0f, surface_b = 0.0f;
@@ -467,22 +479,22 @@ __device__ __forceinline__ void walk_twin(const TileDesc &d, uint32_t rel, bool 
     const uint32_t pov_bytes = pov * static_cast<uint32_t>(sizeof(H));
     const float h0a = static_cast<float>(global_ptr(elev)[pov]) + d.obs;
     const float h0b = static_cast<float>(global_ptr(elev)[pov + 1u]) + d.obs;
-    const uint32_t n_fast = global_ptr(d.meta)[slot].n_fast;
-    const uint32_t tail_start = global_ptr(d.meta)[slot].tail_start;
-    const uint32_t bias = global_ptr(d.meta)[slot].bias;
+    const uint32_t n_fast = const_ptr(d.meta)[slot].n_fast;
+    const uint32_t tail_start = const_ptr(d.meta)[slot].tail_start;
+    const uint32_t bias = const_ptr(d.meta)[slot].bias;
     const __amdgpu_buffer_rsrc_t raw = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<H *>(elev - bias), 0,
         static_cast<int>((bias + d.w * d.w) * sizeof(H)), kHeightRsrcFlags<float>);
 
     const size_t tb = static_cast<size_t>(slot) * d.stride;
-    const TVS_GLOBAL uint32_t *__restrict__ pos0 =
-        global_ptr(((BOTH || !backward) ? d.pos_f : d.pos_b) + tb);
-    const TVS_GLOBAL uint32_t *__restrict__ pos1 = global_ptr(d.pos_b + tb);
-    const TVS_GLOBAL float *__restrict__ dist = global_ptr(d.dist + tb);
-    const TVS_GLOBAL float *__restrict__ rcp = global_ptr(d.rcp + tb);
-    const TVS_GLOBAL float *__restrict__ rcp_lo = global_ptr(d.rcp_lo + tb);
-    const TVS_GLOBAL float *__restrict__ curv = global_ptr(d.curv + tb);
-    const TVS_GLOBAL float *__restrict__ wt = global_ptr(d.wt + tb);
+    const TVS_CONST uint32_t *__restrict__ pos0 =
+        const_ptr(((BOTH || !backward) ? d.pos_f : d.pos_b) + tb);
+    const TVS_CONST uint32_t *__restrict__ pos1 = const_ptr(d.pos_b + tb);
+    const TVS_CONST float *__restrict__ dist = const_ptr(d.dist + tb);
+    const TVS_CONST float *__restrict__ rcp = const_ptr(d.rcp + tb);
+    const TVS_CONST float *__restrict__ rcp_lo = const_ptr(d.rcp_lo + tb);
+    const TVS_CONST float *__restrict__ curv = const_ptr(d.curv + tb);
+    const TVS_CONST float *__restrict__ wt = const_ptr(d.wt + tb);
     const uint64_t lanes = __builtin_amdgcn_read_exec();
 
     float max_angle[NB], surface[NB];
@@ -491,15 +503,6 @@ __device__ __forceinline__ void walk_twin(const TileDesc &d, uint32_t rel, bool 
         max_angle[b] = kMaxAngle;
         surface[b] = 0.0f;
     }
-    auto step = [&](const float (&h)[NB], uint32_t k, bool fast) {
-        const float dk = dist[k];
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const float dh = h[b] - ((b & 1) ? h0b : h0a);
-            const float q = fast ? step_div<DIV>(dh, dk, rcp[k], rcp_lo[k]) : dh / dk;
-            step_update(q - curv[k], wt[k], lanes, max_angle[b], surface[b]);
-        }
-    };
     uint32_t k = 0;
     const uint32_t fast_end = n_fast - n_fast % UNROLL;
     for (; k < fast_end; k += UNROLL) {
@@ -510,13 +513,32 @@ __device__ __forceinline__ void walk_twin(const TileDesc &d, uint32_t rel, bool 
             if (BOTH) load_pair<H>(raw, pov_bytes, pos1[k + u], h[u][NB - 2], h[u][NB - 1]);
         }
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) step(h[u], k + u, DIV != 0);
+        for (int u = 0; u < UNROLL; ++u) {
+            const float dk = dist[k + u];
+            const float y_hi = rcp[k + u];
+            const float y_lo = rcp_lo[k + u];
+            const float c = curv[k + u];
+            const float w = wt[k + u];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const float dh = h[u][b] - ((b & 1) ? h0b : h0a);
+                step_update(step_div<DIV>(dh, dk, y_hi, y_lo) - c, w, lanes, max_angle[b],
+                            surface[b]);
+            }
+        }
     }
     for (; k < tail_start; ++k) {
         float h[NB];
         load_pair<H>(raw, pov_bytes, pos0[k], h[0], h[1]);
         if (BOTH) load_pair<H>(raw, pov_bytes, pos1[k], h[NB - 2], h[NB - 1]);
-        step(h, k, DIV != 0 && k < n_fast);
+        const float dk = dist[k];
+        const bool fast_k = DIV != 0 && k < n_fast;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const float dh = h[b] - ((b & 1) ? h0b : h0a);
+            const float q = fast_k ? step_div<DIV>(dh, dk, rcp[k], rcp_lo[k]) : dh / dk;
+            step_update(q - curv[k], wt[k], lanes, max_angle[b], surface[b]);
+        }
     }
     if (active) {
         TVS_GLOBAL float *out =
@@ -546,7 +568,7 @@ __device__ __forceinline__ uint32_t find_tile(const uint32_t *__restrict__ begin
 // heights passed the range check, else IEEE.
 __device__ __forceinline__ int divide_of(const TileDesc &d, uint32_t rel) {
     if (!d.fast) return 0;
-    if (d.div3_bad && global_ptr(d.div3_bad)[d.first_slot + rel] == 0u) return 3;
+    if (d.div3_bad && const_ptr(d.div3_bad)[d.first_slot + rel] == 0u) return 3;
     return 4;
 }
 
