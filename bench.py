@@ -38,8 +38,9 @@ HBM_PEAK_GBPS = 8000.0
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 # Vector instructions per real band-step of the batched walk's unrolled
 # loop: sub, the divide (4 operations, or 3 with the verified short divide),
-# sub, cmpx, add, max (tests/test_codegen.py holds the compiled loops to
-# these counts).
+# sub, cmpx, add, max, plus one conversion when two points per lane share
+# a dword load (tests/test_codegen.py holds the compiled loops to these
+# counts).
 VALU_PER_STEP = {"split": 9, "short": 8}
 
 
@@ -119,7 +120,7 @@ def measure_large(steps: int, warmup: int) -> dict:
         e.upload(wl.elevations)
         e.compute_bands(0, 180)
         bands_ms = e.timing().bands_ms
-        shape, _ = e.autotune(0, 36, reps=1)
+        shape, _ = e.autotune(0, 180, reps=1)
         ms, kernel_ms, tm = time_method(e, "batched", steps, warmup)
     ps = 180 * 2 * g.computable_points_count * g.real_steps
     nominal = 180 * 2 * g.computable_points_count * g.band_deltas_size
@@ -131,7 +132,8 @@ def measure_large(steps: int, warmup: int) -> dict:
             "bands_ms": bands_ms,
             "achieved_GBps": ps * 2 / (kernel_ms / 1000.0) / 1e9,
             "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous", "pipeline",
-                                                 "persistent", "both_directions")}}
+                                                 "persistent", "both_directions",
+                                                 "two_points")}}
 
 
 def measure_tiles(steps: int, warmup: int) -> dict:
@@ -338,11 +340,11 @@ def main() -> int:
                             wl.observer_height, 0) as ef:
             ef.upload(wl.elevations.astype(np.float32))
             ef.compute_bands(0, 180)
-            ef.set_walk_shape(shape["block"], shape["unroll"], shape["xcd_contiguous"],
-                              shape["pipeline"], shape["persistent"], shape["both_directions"],
-                              shape["two_points"])
+            f_shape, _ = ef.autotune(0, 180, reps=1)   # f32 heights: their own best shape
             f_ms, f_kernel_ms, _ = time_method(ef, "batched", args.steps, args.warmup)
         batched_f32 = {"value": real_point_steps / (f_ms / 1000.0), "ms_per_step": f_ms,
+                       "walk_shape": {k: f_shape[k] for k in ("block", "unroll", "both_directions",
+                                                              "two_points")},
                        "nominal_value": point_steps / (f_ms / 1000.0),
                        "kernel_ms": f_kernel_ms,
                        "achieved_GBps": real_point_steps * 4 / (f_kernel_ms / 1000.0) / 1e9}
@@ -364,6 +366,8 @@ def main() -> int:
     bytes_per_launch = real_point_steps * 2     # one int16 height per executed step
     achieved = bytes_per_launch / (b_kernel_ms / 1000.0) / 1e9
     per_step = VALU_PER_STEP["short"] if short_sectors == 180 else VALU_PER_STEP["split"]
+    if shape["two_points"]:
+        per_step += 1   # the two heights of a dword are converted by one instruction each
     valu_lane_ops = real_point_steps * per_step / (b_kernel_ms / 1000.0)
     traffic = None
     prof = os.path.join(ROOT, "profiles", "walk_tiles_main-1201_traffic.json")
