@@ -101,6 +101,7 @@ def cpu_baseline(wl, sectors: int, threads: int):
             "kind": "port",
             "bands_ms_per_sector": bands_s * 1000.0 / len(picked),
             "bands_s": bands_s, "walk_s": dt,
+            "walk_s_180_sectors": dt * 180.0 / len(picked),   # extrapolated when sampled
             "sample": ((f"{len(picked)} of 180 sectors (every {180 // len(picked)}th)"
                         if len(picked) < 180 else "all 180 sectors") + f" of "
                        f"{wl.name}, walk only (every step, zero tail included), oracle C "
@@ -109,8 +110,11 @@ def cpu_baseline(wl, sectors: int, threads: int):
                        f"separately, {threads} thread(s)")}
 
 
-def measure_large(steps: int, warmup: int) -> dict:
-    """3601 x 3601 at 30 m, LoS 40 km (1333 steps per band), int16."""
+def measure_large(steps: int, warmup: int, cpu: bool = False) -> dict:
+    """3601 x 3601 at 30 m, LoS 40 km (1333 steps per band), int16.  With
+    `cpu`, also the oracle walk on one core over every 10th sector
+    (extrapolated to 180 by the executed-step rate) and on every host core
+    over all 180 sectors."""
     import tvs_amd
     wl = workload("large-3601")
     log(f"large tile {wl.width}x{wl.width}: bands + tune + walk")
@@ -125,15 +129,21 @@ def measure_large(steps: int, warmup: int) -> dict:
     ps = 180 * 2 * g.computable_points_count * g.real_steps
     nominal = 180 * 2 * g.computable_points_count * g.band_deltas_size
     log(f"large: {ms:.2f} ms/step, {ps / ms * 1e3:.3e} executed point-steps/s")
-    return {"workload": wl.name, "value": ps / (ms / 1000.0), "unit": "point-steps/s",
-            "nominal_value": nominal / (ms / 1000.0),
-            "ms_per_step": ms, "kernel_ms": kernel_ms, "reduce_ms": tm.reduce_ms,
-            "point_steps_per_step": ps, "nominal_point_steps_per_step": nominal,
-            "bands_ms": bands_ms,
-            "achieved_GBps": ps * 2 / (kernel_ms / 1000.0) / 1e9,
-            "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous", "pipeline",
-                                                 "persistent", "both_directions",
-                                                 "two_points")}}
+    res = {"workload": wl.name, "value": ps / (ms / 1000.0), "unit": "point-steps/s",
+           "nominal_value": nominal / (ms / 1000.0),
+           "ms_per_step": ms, "kernel_ms": kernel_ms, "reduce_ms": tm.reduce_ms,
+           "point_steps_per_step": ps, "nominal_point_steps_per_step": nominal,
+           "bands_ms": bands_ms,
+           "achieved_GBps": ps * 2 / (kernel_ms / 1000.0) / 1e9,
+           "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous", "pipeline",
+                                                "persistent", "both_directions", "two_points")}}
+    if cpu:
+        log("large tile: CPU oracle walk, 1 thread, every 10th sector")
+        res["cpu_one_core"] = cpu_baseline(wl, 18, 1)
+        cores = min(16, len(os.sched_getaffinity(0)))
+        log(f"large tile: CPU oracle walk, {cores} threads, all 180 sectors")
+        res["cpu_all_cores"] = cpu_baseline(wl, 180, cores)
+    return res
 
 
 def measure_tiles(steps: int, warmup: int) -> dict:
@@ -262,6 +272,8 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the large-3601 and tiles-64 figures")
     ap.add_argument("--cpu-sectors", type=int, default=36)
+    ap.add_argument("--cpu-large", action="store_true",
+                    help="also time the CPU oracle on the 3601x3601 tile (a few minutes)")
     ap.add_argument("--cpu-threads", type=int, default=1)
     ap.add_argument("--copy-calibration", type=int, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration)")
@@ -434,7 +446,7 @@ def main() -> int:
         "bands_ms": bands_ms,
     }
     if not args.no_extra:
-        out["extra"] = [measure_large(max(2, args.steps // 2), 1),
+        out["extra"] = [measure_large(max(2, args.steps // 2), 1, args.cpu_large),
                         measure_tiles(max(2, args.steps // 2), 1)]
     if not args.no_cpu_baseline:
         log(f"CPU baseline: oracle walk over {args.cpu_sectors} sectors, "
