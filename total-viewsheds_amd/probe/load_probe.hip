@@ -6,8 +6,8 @@
 // from a buffer that stays in cache.  Reported: nanoseconds per wave-load
 // instruction per CU for a 16-bit typed load (the one-point walk's load), a
 // 16-bit raw load, a 4-byte-aligned dword (two heights, the two-point walk's
-// load when aligned), a dword at 2 mod 4 (misaligned) and a two-component
-// typed load.
+// load when aligned), a dword at 2 mod 4 (misaligned), a two-component
+// typed load, and a dwordx2 (two f32 heights) aligned and at 4 mod 8.
 //
 // Build: make -C total-viewsheds_amd probe.  Run: total-viewsheds_amd/load_probe
 #include <hip/hip_runtime.h>
@@ -26,7 +26,7 @@ __device__ tvs_probe_float2 tvs_probe_load_format_xy(__amdgpu_buffer_rsrc_t rsrc
 namespace {
 
 constexpr int kBlock = 256;
-constexpr uint32_t kSpanBytes = 1u << 21;   // offsets stay below this
+constexpr uint32_t kSpanBytes = 1u << 21;   // scalar offsets stay below this
 constexpr int kFormat16 = 4 | (3 << 12) | (2 << 15);             // X, SSCALED, 16
 constexpr int kFormat16x2 = 4 | (5 << 3) | (3 << 12) | (5 << 15); // XY, SSCALED, 16_16
 constexpr int kRaw = 0x00020000;
@@ -37,7 +37,7 @@ template <int MODE>
 __global__ __launch_bounds__(kBlock) void load_kernel(const int16_t *buf, float *out, int iters) {
     const int flags = MODE == 0 ? kFormat16 : MODE == 4 ? kFormat16x2 : kRaw;
     const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<int16_t *>(buf), 0, static_cast<int>(kSpanBytes + 512), flags);
+        const_cast<int16_t *>(buf), 0, static_cast<int>(kSpanBytes + 1024), flags);
     const uint32_t lane = threadIdx.x & 63u;
     float acc = 0.0f;
     for (int i = 0; i < iters; ++i) {
@@ -57,6 +57,10 @@ __global__ __launch_bounds__(kBlock) void load_kernel(const int16_t *buf, float 
             if (MODE == 4) {
                 const tvs_probe_float2 t = tvs_probe_load_format_xy(r, lane * 4, so, 0);
                 acc += t.x + t.y;
+            }
+            if (MODE == 5 || MODE == 6) {   // dwordx2 (two f32 heights), 8-aligned / 4 mod 8
+                const auto t = __builtin_amdgcn_raw_buffer_load_b64(r, lane * 8, so + (MODE == 6 ? 4u : 0u), 0);
+                acc += __uint_as_float((t[0] ^ t[1]) & 0x3fffffffu);
             }
         }
     }
@@ -103,6 +107,8 @@ int main() {
     rc |= run<2>(buf, out, blocks, cus, 2000, "dword, 4-byte aligned (two heights)");
     rc |= run<3>(buf, out, blocks, cus, 2000, "dword at 2 mod 4 (two heights, misaligned)");
     rc |= run<4>(buf, out, blocks, cus, 2000, "typed 16_16 (format_xy, two heights)");
+    rc |= run<5>(buf, out, blocks, cus, 2000, "dwordx2, 8-byte aligned (two f32 heights)");
+    rc |= run<6>(buf, out, blocks, cus, 2000, "dwordx2 at 4 mod 8 (two f32 heights)");
     (void)hipFree(buf);
     (void)hipFree(out);
     return rc;
