@@ -192,6 +192,83 @@ def measure_tiles(steps: int, warmup: int) -> dict:
                                                  "two_points")}}
 
 
+def measure_next_rows(wl) -> dict:
+    """The paths beside the walk, on the same workload: `.bt` write/read and
+    the CLI end to end (input file to heatmap), the band-data cache for all
+    180 sectors, one sector's compact ring export, the heatmap PNG.  Each
+    result is checked against its source while it is timed."""
+    import subprocess
+    import tempfile
+    import tvs_amd
+    from tvs_amd import bt
+    w = wl.width
+    res = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "dem.bt")
+        t0 = time.perf_counter()
+        bt.write(path, wl.elevations.reshape(w, w))
+        res["bt_write_ms"] = (time.perf_counter() - t0) * 1000.0
+        t0 = time.perf_counter()
+        back = bt.read(path)
+        res["bt_read_ms"] = (time.perf_counter() - t0) * 1000.0
+        assert np.array_equal(back, wl.elevations), ".bt round trip"
+        res["bt_bytes"] = os.path.getsize(path)
+        state = os.path.join(tmp, "state")
+        with tvs_amd.Engine(w, wl.spacing, wl.max_line_of_sight, wl.rings_per_km,
+                            wl.observer_height, 0) as e:
+            e.upload(wl.elevations)
+            e.compute_bands(0, 180)
+            t0 = time.perf_counter()
+            for s in range(180):
+                e.cache_save(state, s)
+            res["cache_save_ms_180"] = (time.perf_counter() - t0) * 1000.0
+            with tvs_amd.Engine(w, wl.spacing, wl.max_line_of_sight, wl.rings_per_km,
+                                wl.observer_height, 0) as e2:
+                e2.upload(wl.elevations)
+                t0 = time.perf_counter()
+                for s in range(180):
+                    e2.cache_load(state, s)
+                res["cache_load_ms_180"] = (time.perf_counter() - t0) * 1000.0
+                for s in range(0, 180, 7):
+                    a, b = e.get_band(s), e2.get_band(s)
+                    assert np.array_equal(a[0], b[0]) and np.array_equal(
+                        a[1].view(np.uint32), b[1].view(np.uint32)), f"cache sector {s}"
+            e.compact_rings(46)                       # first call allocates
+            t0 = time.perf_counter()
+            counts, events = e.compact_rings(46)
+            res["ring_export_ms_one_sector"] = (time.perf_counter() - t0) * 1000.0
+            assert np.all(counts % 2 == 0), "ring counts are even"
+            bands = 2 * e.n_points
+            res["ring_events_one_sector"] = int(events.size)
+            res["ring_export_bytes"] = int(4 * (counts.size + events.size) + 8 * bands)
+            res["ring_fixed_slot_bytes"] = int(4 * bands * tvs_amd.ring_count_per_band(
+                wl.rings_per_km, wl.max_line_of_sight))
+            e.run(0, 180, "batched")
+            T = e.geometry.tvs_width
+            surf = e.surfaces().reshape(T, T)
+            png = os.path.join(tmp, "heatmap.png")
+            t0 = time.perf_counter()
+            bt.heatmap_png(png, surf)
+            res["png_ms"] = (time.perf_counter() - t0) * 1000.0
+            res["png_bytes"] = os.path.getsize(png)
+        cli = os.path.join(ROOT, "total-viewsheds_amd", "total-viewsheds")
+        out_dir = os.path.join(tmp, "cli")
+        os.makedirs(out_dir)
+        t0 = time.perf_counter()
+        subprocess.run([cli, "--input", path, "--max-line-of-sight", str(wl.max_line_of_sight),
+                        "--rings-per-km", str(wl.rings_per_km), "--observer-height",
+                        str(wl.observer_height), "--scale", str(wl.spacing), "--output-dir",
+                        out_dir, "--no-cache"], check=True, capture_output=True, timeout=300)
+        res["cli_end_to_end_ms"] = (time.perf_counter() - t0) * 1000.0
+        assert os.path.getsize(os.path.join(out_dir, "heatmap.png")) > 0
+    log(f"next rows: .bt write {res['bt_write_ms']:.1f} ms / read {res['bt_read_ms']:.1f} ms, "
+        f"cache save {res['cache_save_ms_180']:.1f} ms / load {res['cache_load_ms_180']:.1f} ms "
+        f"(180 sectors), ring export {res['ring_export_ms_one_sector']:.1f} ms "
+        f"({res['ring_events_one_sector']} events), PNG {res['png_ms']:.1f} ms, "
+        f"CLI end to end {res['cli_end_to_end_ms']:.0f} ms")
+    return res
+
+
 def copy_calibration(mib: int) -> int:
     """A plain device-to-device copy of known size (the runtime's copy
     kernel), to scale the FETCH_SIZE / WRITE_SIZE counters."""
@@ -446,6 +523,7 @@ def main() -> int:
         "bands_ms": bands_ms,
     }
     if not args.no_extra:
+        out["next_rows"] = measure_next_rows(wl)
         out["extra"] = [measure_large(max(2, args.steps // 2), 1, args.cpu_large),
                         measure_tiles(max(2, args.steps // 2), 1)]
     if not args.no_cpu_baseline:
