@@ -671,9 +671,31 @@ __global__ __launch_bounds__(kBlock) void reduce_tiles_kernel(
     TVS_GLOBAL double *cum64 = global_ptr(d.cum64);
     float c = cum[t];
     double s = cum64[t];
-    for (uint32_t r = 0; r < d.count; ++r) {
-        const float f = partial[static_cast<size_t>(2u * r) * N + t];
-        const float b = partial[static_cast<size_t>(2u * r + 1u) * N + t];
+    // Loads for kReduceChunk sectors are issued before their adds (the adds
+    // stay in sector order): the kernel is bound by HBM latency otherwise,
+    // with only a few waves per SIMD at one point per thread.
+    constexpr uint32_t kReduceChunk = 16;
+    const size_t stride = 2ull * N;
+    const TVS_GLOBAL float *__restrict__ p = partial + t;
+    uint32_t r = 0;
+    for (; r + kReduceChunk <= d.count; r += kReduceChunk) {
+        float f[kReduceChunk], b[kReduceChunk];
+#pragma unroll
+        for (uint32_t u = 0; u < kReduceChunk; ++u) {
+            f[u] = p[(r + u) * stride];
+            b[u] = p[(r + u) * stride + N];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kReduceChunk; ++u) {
+            c = c + f[u];
+            c = c + b[u];
+            s = s + static_cast<double>(f[u]);
+            s = s + static_cast<double>(b[u]);
+        }
+    }
+    for (; r < d.count; ++r) {
+        const float f = p[r * stride];
+        const float b = p[r * stride + N];
         c = c + f;
         c = c + b;
         s = s + static_cast<double>(f);
