@@ -2,7 +2,7 @@
 hipcc cross-compiles).  Guards the properties the kernel's speed rests on:
 8 height loads with scalar offsets per chunk (typed loads that convert the
 int16 heights, so no conversion instruction), the step tables read with
-scalar dwordx8 loads, the exec-masked visibility update, one FMA fewer per
+scalar dwordx8 loads, the exec-masked visibility update, two FMAs fewer per
 step in the short-divide loop, and no divergent branch in the loop."""
 import os
 import re
@@ -70,36 +70,36 @@ def check_loop(loop: str, load: str, band_steps: int) -> None:
 @pytest.mark.parametrize("pipe", ["Lb0E", "Lb1E"])
 def test_walk_loop_shape(asm, heights, load, pipe):
     """One band per lane, 8 steps per chunk: the split-reciprocal loop spends
-    9 VALU per step, the short-divide loop 8, the IEEE loop more."""
+    9 VALU per step, the short-divide loop 7, the IEEE loop more."""
     loops = unrolled_loops(kernel_body(asm, heights + "Lb0ELi256ELi8E" + pipe + "Lb0ELb0ELb0E"))
     fast = [l for l in loops if "v_div_scale" not in l]
     assert len(fast) == 2
     for loop in fast:
         check_loop(loop, load, 8)
     counts = sorted(valu(l) for l in fast)
-    assert counts[0] <= 8 * 8 + 8 and counts[1] <= 9 * 8 + 8, counts
-    assert counts[1] - counts[0] >= 8                    # one FMA fewer per step
+    assert counts[0] <= 7 * 8 + 8 and counts[1] <= 9 * 8 + 8, counts
+    assert counts[1] - counts[0] >= 16                   # two FMAs fewer per step
 
 
 def test_paired_walk_loop_shape(asm):
     """Both bands of a point per lane: 16 height loads and 16 masked updates
     per 8-step chunk, one set of table reads (pos_f, pos_b + the shared
-    constants), 9 (split reciprocal) or 8 (short divide) VALU per band-step."""
+    constants), 9 (split reciprocal) or 7 (short divide) VALU per band-step."""
     loops = unrolled_loops(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb1ELb0E"))
     fast = [l for l in loops if "v_div_scale" not in l]
     assert len(fast) == 2
     for loop in fast:
         check_loop(loop, "buffer_load_format_x", 16)
     short, split = sorted(fast, key=valu)
-    assert valu(short) <= 8 * 16 + 8 and valu(split) <= 9 * 16 + 8, (valu(short), valu(split))
+    assert valu(short) <= 7 * 16 + 8 and valu(split) <= 9 * 16 + 8, (valu(short), valu(split))
     assert len(re.findall(r"s_load_dwordx8", split)) == 7
-    assert len(re.findall(r"s_load_dwordx8", short)) == 6      # no low reciprocal part
+    assert len(re.findall(r"s_load_dwordx8", short)) == 6      # no distance needed
 
 
 def test_two_point_walk_loop_shape(asm):
     """Two adjacent points per lane: one dword load per step fetches both
     int16 heights (8 loads for 16 band-steps), each converted by one SDWA
-    instruction; 9 (short divide) or 10 VALU per band-step."""
+    instruction; 8 (short divide) or 10 VALU per band-step."""
     loops = unrolled_loops(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb0ELb1E"))
     twin = [l for l in loops if "v_div_scale" not in l and "buffer_load_dword" in l]
     assert len(twin) == 2
@@ -111,7 +111,7 @@ def test_two_point_walk_loop_shape(asm):
         assert len(re.findall(r"_sdwa", loop)) == 16
         assert "scratch" not in loop and "buffer_store" not in loop
     short, split = sorted(twin, key=valu)
-    assert valu(short) <= 9 * 16 + 8 and valu(split) <= 10 * 16 + 8, (valu(short), valu(split))
+    assert valu(short) <= 8 * 16 + 8 and valu(split) <= 10 * 16 + 8, (valu(short), valu(split))
 
 
 def test_no_packed_fp32_and_no_spills(asm):

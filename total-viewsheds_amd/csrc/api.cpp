@@ -197,7 +197,7 @@ tvs_status upload_common(tvs_ctx *c, bool integral, int64_t lo, int64_t hi) {
                      fast_divide_holds(lo, hi, c->params.observer_height, &mn, &mx);
     c->min_abs_dh = mn;
     c->max_abs_dh = mx;
-    std::fill(c->div3_checked.begin(), c->div3_checked.end(), 0);
+    std::fill(c->short_div_checked.begin(), c->short_div_checked.end(), 0);
     dev_free(c->div_domain);
     c->div_domain_n = 0;
     if (!c->fast_div_ok) return TVS_OK;
@@ -232,21 +232,21 @@ tvs_status validate_slots(tvs_ctx *c, uint32_t first, uint32_t count) {
 // Verifies the short divide for the unchecked slots of [first, first +
 // count) of context c against step tables `tab` (c's own, or the band
 // owner's in a batch).  No-op when the short divide is off or unavailable.
-tvs_status ensure_div3(tvs_ctx *c, const StepTables &tab, uint32_t first, uint32_t count) {
+tvs_status ensure_short_div(tvs_ctx *c, const StepTables &tab, uint32_t first, uint32_t count) {
     if (!c->short_div || !c->fast_div_ok || !c->div_domain) return TVS_OK;
     uint32_t s = first;
     while (s < first + count) {
-        if (c->div3_checked[s]) {
+        if (c->short_div_checked[s]) {
             ++s;
             continue;
         }
         uint32_t e = s;
-        while (e < first + count && !c->div3_checked[e]) ++e;
-        TVS_HIP(c, launch_fill_u32(c->stream, c->div3_bad + s, e - s, 0u), "divide check");
-        TVS_HIP(c, launch_div3_check(c->stream, tab, s, e - s, c->div_domain,
-                                     c->div_domain_n, c->div3_bad),
+        while (e < first + count && !c->short_div_checked[e]) ++e;
+        TVS_HIP(c, launch_fill_u32(c->stream, c->short_div_bad + s, e - s, 0u), "divide check");
+        TVS_HIP(c, launch_short_div_check(c->stream, tab, s, e - s, c->div_domain,
+                                     c->div_domain_n, c->short_div_bad),
                 "divide check");
-        for (uint32_t k = s; k < e; ++k) c->div3_checked[k] = 1;
+        for (uint32_t k = s; k < e; ++k) c->short_div_checked[k] = 1;
         s = e;
     }
     return TVS_OK;
@@ -304,7 +304,7 @@ tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t cap
     d.R = c->rings_per_band;
     d.capture_slots = capture;
     d.fast = c->fast_div_ok ? 1u : 0u;
-    d.div3_bad = (c->short_div && c->fast_div_ok && c->div_domain) ? c->div3_bad : nullptr;
+    d.short_div_bad = (c->short_div && c->fast_div_ok && c->div_domain) ? c->short_div_bad : nullptr;
     d.ring_mode = kRingRecords;
     d.ring_counts = nullptr;
     d.ring_offsets = nullptr;
@@ -397,7 +397,7 @@ tvs_status tvs_create(const tvs_params *p, tvs_ctx **out) {
     if (e == hipSuccess) e = dev_alloc(&c->tab.curv, tab_n);
     if (e == hipSuccess) e = dev_alloc(&c->tab.wt, tab_n);
     if (e == hipSuccess) e = dev_alloc(&c->tab.meta, tvs::kBandSlots);
-    if (e == hipSuccess) e = dev_alloc(&c->div3_bad, tvs::kBandSlots);
+    if (e == hipSuccess) e = dev_alloc(&c->short_div_bad, tvs::kBandSlots);
     if (e == hipSuccess) e = hipMemsetAsync(c->cum, 0, sizeof(float) * std::max(N, 1u), c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->cum64, 0, sizeof(double) * std::max(N, 1u), c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->overflow, 0, sizeof(unsigned long long), c->stream);
@@ -408,7 +408,7 @@ tvs_status tvs_create(const tvs_params *p, tvs_ctx **out) {
         return s2;
     }
     c->band_ready.assign(tvs::kBandSlots, 0);
-    c->div3_checked.assign(tvs::kBandSlots, 0);
+    c->short_div_checked.assign(tvs::kBandSlots, 0);
     c->meta_host.assign(tvs::kBandSlots, tvs::SectorMeta{});
     *out = c;
     return TVS_OK;
@@ -429,7 +429,7 @@ tvs_status tvs_destroy(tvs_ctx *c) {
     dev_free(c->tab.curv);
     dev_free(c->tab.wt);
     dev_free(c->tab.meta);
-    dev_free(c->div3_bad);
+    dev_free(c->short_div_bad);
     dev_free(c->div_domain);
     dev_free(c->cum);
     dev_free(c->cum64);
@@ -536,7 +536,7 @@ tvs_status tvs_compute_bands(tvs_ctx *c, uint32_t first, uint32_t count) {
     }
     c->bands_sorted_sectors = static_cast<uint32_t>(sorted_path.size());
     TVS_HIP(c, tvs::launch_step_tables(c, first, count), "step tables");
-    std::fill(c->div3_checked.begin() + first, c->div3_checked.begin() + first + count, 0);
+    std::fill(c->short_div_checked.begin() + first, c->short_div_checked.begin() + first + count, 0);
     TVS_HIP(c, hipEventRecord(c->ev[5], c->stream), "event");
     TVS_HIP(c, hipEventSynchronize(c->ev[5]), "band computation");
     float ms = 0.0f;
@@ -559,11 +559,11 @@ tvs_status tvs_get_short_divide(tvs_ctx *c, uint32_t *enabled, uint32_t *verifie
         if (c->short_div && c->fast_div_ok && c->div_domain) {
             std::vector<uint32_t> bad(tvs::kSectorSteps, 1u);
             TVS_HIP(c, hipStreamSynchronize(c->stream), "sync");
-            TVS_HIP(c, hipMemcpy(bad.data(), c->div3_bad, bad.size() * sizeof(uint32_t),
+            TVS_HIP(c, hipMemcpy(bad.data(), c->short_div_bad, bad.size() * sizeof(uint32_t),
                                  hipMemcpyDeviceToHost),
                     "divide check readback");
             for (uint32_t s = 0; s < tvs::kSectorSteps; ++s)
-                if (c->div3_checked[s] && bad[s] == 0u) ++n;
+                if (c->short_div_checked[s] && bad[s] == 0u) ++n;
         }
         *verified_sectors = n;
     }
@@ -625,7 +625,7 @@ static tvs_status set_band_slot(tvs_ctx *c, uint32_t slot, const int32_t *deltas
                 "band upload");
     }
     TVS_HIP(c, tvs::launch_step_tables(c, slot, 1), "step tables");
-    c->div3_checked[slot] = 0;
+    c->short_div_checked[slot] = 0;
     return read_meta(c, slot, 1);
 }
 
@@ -700,7 +700,7 @@ tvs_status tvs_run_sectors(tvs_ctx *c, uint32_t first, uint32_t count, tvs_metho
         tvs::TileDesc d;
         if ((st = tvs::prepare_tile(c, first, count, capture, cfg, &d)) != TVS_OK)
             return st;
-        if ((st = tvs::ensure_div3(c, c->tab, first, count)) != TVS_OK) return st;
+        if ((st = tvs::ensure_short_div(c, c->tab, first, count)) != TVS_OK) return st;
         const uint32_t n_items = d.blocks_per_pass * 2u * count;
         const uint32_t begin[4] = {0u, n_items, 0u, N};
         TVS_HIP(c, hipMemcpyAsync(c->d_tile, &d, sizeof d, hipMemcpyHostToDevice, c->stream),

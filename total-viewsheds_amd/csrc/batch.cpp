@@ -19,7 +19,7 @@
 namespace tvs {
 tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t capture,
                         const WalkConfig &cfg, TileDesc *out);
-tvs_status ensure_div3(tvs_ctx *c, const StepTables &tab, uint32_t first, uint32_t count);
+tvs_status ensure_short_div(tvs_ctx *c, const StepTables &tab, uint32_t first, uint32_t count);
 }
 
 struct tvs_batch {
@@ -99,7 +99,7 @@ tvs_status prepare(tvs_batch *b) {
         tvs_status st = tvs::prepare_tile(c, 0, tvs::kSectorSteps, 0, b->cfg, &d);
         if (st != TVS_OK) return from_tile(b, i, st);
         // The short divide is checked against the tables the tile walks with.
-        st = tvs::ensure_div3(c, o->tab, 0, tvs::kSectorSteps);
+        st = tvs::ensure_short_div(c, o->tab, 0, tvs::kSectorSteps);
         if (st != TVS_OK) return from_tile(b, i, st);
         d.off = o->tab.off;
         d.pos_f = o->tab.pos_f;
@@ -240,7 +240,7 @@ tvs_status tvs_batch_compute_bands(tvs_batch *b) {
     for (uint32_t i = 0; i < b->tiles.size(); ++i) {
         const uint32_t o = b->owner[i];
         tvs_ctx *c = b->tiles[i];
-        std::fill(c->div3_checked.begin(), c->div3_checked.end(), 0);   // new tables
+        std::fill(c->short_div_checked.begin(), c->short_div_checked.end(), 0);   // new tables
         if (o == i) continue;
         c->meta_host = b->tiles[o]->meta_host;
         const tvs_status st = tvs::validate_slots(c, 0, tvs::kSectorSteps);

@@ -83,7 +83,7 @@ struct TileDesc {
     uint32_t blocks_per_pass;    // ceil(N / block)
     uint32_t R, capture_slots;
     uint32_t fast;               // heights passed the fast-divide range check
-    const uint32_t *div3_bad;    // per slot: 0 = the short divide was verified for
+    const uint32_t *short_div_bad;    // per slot: 0 = the short divide was verified for
                                  // every height difference of this DEM (null: off)
     uint32_t ring_mode;          // kRingRecords / kRingCount / kRingCompact
     uint32_t *ring_counts;       // u32[2N]: ring_id per band (kRingCount)
@@ -175,8 +175,8 @@ struct tvs_ctx {
     bool short_div = true;                 // caller's switch (default on)
     float *div_domain = nullptr;
     uint32_t div_domain_n = 0;
-    uint32_t *div3_bad = nullptr;          // u32[kSlots]
-    std::vector<uint8_t> div3_checked;     // per slot
+    uint32_t *short_div_bad = nullptr;          // u32[kSlots]
+    std::vector<uint8_t> short_div_checked;     // per slot
 
     tvs::StepTables tab;
     std::vector<uint8_t> band_ready;   // per slot: band data resident
@@ -223,7 +223,7 @@ bool walk_shape_supported(const WalkConfig &cfg);
 // Per slot in [first, first + count): bad[slot] = 1 if the short divide
 // differs from the split-reciprocal divide for any value of `domain` at any
 // regular step of the slot's table (bad must be zero on entry).
-hipError_t launch_div3_check(hipStream_t s, const StepTables &tab, uint32_t first,
+hipError_t launch_short_div_check(hipStream_t s, const StepTables &tab, uint32_t first,
                              uint32_t count, const float *domain, uint32_t n_domain,
                              uint32_t *bad);
 // Every compiled shape the tuners time: both item mappings, persistent

@@ -43,15 +43,15 @@ TVS_HD float div_by_reciprocal(float a, float d, float y_hi, float y_lo) {
     return std::fma(r, y_hi, q0);
 }
 
-// Three operations: q0 = RN(a*y_hi), then the same exact-residual step.
-// Not proven correctly rounded for every (a, d) -- q0 may miss a/d by more
-// than one ulp -- so the walk uses it for a sector only after the device has
-// compared it with div_by_reciprocal for every height difference the DEM
-// can produce at every regular step of that sector.
-TVS_HD float div_short(float a, float d, float y_hi) {
-    const float q0 = a * y_hi;
-    const float r = std::fma(-q0, d, a);
-    return std::fma(r, y_hi, q0);
+// Two operations: q0 of div_by_reciprocal without its residual step.  q0 is
+// within 0.5 ulp + 2^-46.4 |a/d| of a/d, so it is the correctly rounded
+// quotient unless a/d lies within ~2^-22 ulp of a rounding midpoint -- which
+// does happen (e.g. a = -1463.65039, d = 126.479172) -- so the walk uses it
+// for a sector only after the device has compared it with div_by_reciprocal
+// for every height difference the DEM can produce at every regular step of
+// that sector.
+TVS_HD float div_short(float a, float y_hi, float y_lo) {
+    return std::fma(a, y_hi, a * y_lo);
 }
 
 // Distances the fast divide is proven for.

@@ -210,12 +210,12 @@ __device__ __forceinline__ const TVS_CONST T *const_ptr(const T *p) {
 }
 
 // The divide of a regular step.  DIV 4: the split-reciprocal divide
-// (proven correctly rounded in the fast-divide range); DIV 3: the short
+// (proven correctly rounded in the fast-divide range); DIV 2: the short
 // divide (used only for sectors verified against DIV 4); DIV 0: IEEE.
 template <int DIV>
 __device__ __forceinline__ float step_div(float a, float d, float y_hi, float y_lo) {
     if (DIV == 4) return div_by_reciprocal(a, d, y_hi, y_lo);
-    if (DIV == 3) return div_short(a, d, y_hi);
+    if (DIV == 2) return div_short(a, y_hi, y_lo);
     return a / d;
 }
 
@@ -568,7 +568,7 @@ __device__ __forceinline__ uint32_t find_tile(const uint32_t *__restrict__ begin
 // heights passed the range check, else IEEE.
 __device__ __forceinline__ int divide_of(const TileDesc &d, uint32_t rel) {
     if (!d.fast) return 0;
-    if (d.div3_bad && const_ptr(d.div3_bad)[d.first_slot + rel] == 0u) return 3;
+    if (d.short_div_bad && const_ptr(d.short_div_bad)[d.first_slot + rel] == 0u) return 2;
     return 4;
 }
 
@@ -601,8 +601,8 @@ __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
             const bool active = p_raw < n_pairs;
             const uint32_t p = active ? p_raw : n_pairs - 1u;
             const uint32_t t0 = (p / half) * d.T + 2u * (p % half);
-            if (div == 3)
-                walk_twin<H, 3, UNROLL, PAIR>(d, rel, backward, t0, active);
+            if (div == 2)
+                walk_twin<H, 2, UNROLL, PAIR>(d, rel, backward, t0, active);
             else if (div == 4)
                 walk_twin<H, 4, UNROLL, PAIR>(d, rel, backward, t0, active);
             else
@@ -618,16 +618,16 @@ __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
     const uint32_t i = active ? i_raw : n_single - 1u;   // spare lanes shadow
     const uint32_t t = TWIN ? i * d.T + (d.T - 1u) : i;
     if (PAIR) {
-        if (div == 3)
-            walk_pair<H, 3, UNROLL>(d, rel, t, active);
+        if (div == 2)
+            walk_pair<H, 2, UNROLL>(d, rel, t, active);
         else if (div == 4)
             walk_pair<H, 4, UNROLL>(d, rel, t, active);
         else
             walk_pair<H, 0, UNROLL>(d, rel, t, active);
         return;
     }
-    if (div == 3)
-        walk_band<H, 3, CAPTURE, UNROLL, PIPE>(d, rel, backward, t, active);
+    if (div == 2)
+        walk_band<H, 2, CAPTURE, UNROLL, PIPE>(d, rel, backward, t, active);
     else if (div == 4)
         walk_band<H, 4, CAPTURE, UNROLL, PIPE>(d, rel, backward, t, active);
     else
@@ -721,18 +721,18 @@ __global__ void widen_i16_kernel(const int16_t *in, float *out, size_t n) {
 // steps); threads stride over the domain of height differences and compare
 // the short divide with the split-reciprocal divide at each of the chunk's
 // steps.  Any difference marks the slot (every writer stores 1).
-constexpr uint32_t kDiv3Chunk = 64;
+constexpr uint32_t kShortDivChunk = 64;
 
-__global__ __launch_bounds__(kBlock) void div3_check_kernel(StepTables tab, uint32_t first,
+__global__ __launch_bounds__(kBlock) void short_div_check_kernel(StepTables tab, uint32_t first,
                                                             const float *__restrict__ domain,
                                                             uint32_t n_domain,
                                                             uint32_t *__restrict__ bad) {
     const uint32_t slot = first + blockIdx.x;
     const uint32_t n_fast = tab.meta[slot].n_fast;
-    const uint32_t k0 = blockIdx.y * kDiv3Chunk;
+    const uint32_t k0 = blockIdx.y * kShortDivChunk;
     if (k0 >= n_fast) return;   // uniform over the block
-    const uint32_t nk = min(n_fast - k0, kDiv3Chunk);
-    __shared__ float sd[kDiv3Chunk], shi[kDiv3Chunk], slo[kDiv3Chunk];
+    const uint32_t nk = min(n_fast - k0, kShortDivChunk);
+    __shared__ float sd[kShortDivChunk], shi[kShortDivChunk], slo[kShortDivChunk];
     const size_t tb = static_cast<size_t>(slot) * tab.stride + k0;
     for (uint32_t k = threadIdx.x; k < nk; k += blockDim.x) {
         sd[k] = tab.dist[tb + k];
@@ -745,8 +745,8 @@ __global__ __launch_bounds__(kBlock) void div3_check_kernel(StepTables tab, uint
         const float a = domain[i];
         for (uint32_t k = 0; k < nk; ++k) {
             const float q4 = div_by_reciprocal(a, sd[k], shi[k], slo[k]);
-            const float q3 = div_short(a, sd[k], shi[k]);
-            differ |= __float_as_uint(q4) ^ __float_as_uint(q3);
+            const float q2 = div_short(a, shi[k], slo[k]);
+            differ |= __float_as_uint(q4) ^ __float_as_uint(q2);
         }
     }
     if (differ) bad[slot] = 1u;
@@ -832,13 +832,13 @@ hipError_t launch_walk_tiles(hipStream_t s, const TileDesc *tiles, const uint32_
                : launch_tiles_h<float>(s, tiles, item_begin, n_tiles, n_items, capture, cfg);
 }
 
-hipError_t launch_div3_check(hipStream_t s, const StepTables &tab, uint32_t first,
+hipError_t launch_short_div_check(hipStream_t s, const StepTables &tab, uint32_t first,
                              uint32_t count, const float *domain, uint32_t n_domain,
                              uint32_t *bad) {
     if (count == 0 || n_domain == 0) return hipSuccess;
-    const uint32_t chunks = (tab.stride + kDiv3Chunk - 1) / kDiv3Chunk;
+    const uint32_t chunks = (tab.stride + kShortDivChunk - 1) / kShortDivChunk;
     if (chunks == 0) return hipSuccess;
-    hipLaunchKernelGGL(div3_check_kernel, dim3(count, chunks), dim3(kBlock), 0, s, tab, first,
+    hipLaunchKernelGGL(short_div_check_kernel, dim3(count, chunks), dim3(kBlock), 0, s, tab, first,
                        domain, n_domain, bad);
     return hipGetLastError();
 }

@@ -1,7 +1,7 @@
 // issue_probe -- the instruction-issue ceiling of the batched walk's step.
 //
 // Runs the walk's per-step vector sequence (sub, mul, fmac, fma, fmac,
-// subrev, v_cmpx, masked add, exec restore, max: 9 VALU + 1 SALU; 8 VALU
+// subrev, v_cmpx, masked add, exec restore, max: 9 VALU + 1 SALU; 7 VALU
 // with the short divide) out of registers, with no memory access, on every SIMD at full occupancy, and
 // reports nanoseconds per wave-instruction per SIMD.  Next to it: the same
 // count of independent FMAs and of independent adds.  The walk kernel's own
@@ -50,7 +50,6 @@ __global__ __launch_bounds__(kBlock) void issue_kernel(float *out, int iters, fl
 #define TVS_STEP3(j)                                                                   \
     asm volatile("v_sub_f32_e32 %0, %0, %2\n\t"                                        \
                  "v_mul_f32_e32 %1, %3, %0\n\t"                                        \
-                 "v_fma_f32 %0, -%1, %3, %0\n\t"                                       \
                  "v_fmac_f32_e32 %1, %4, %0\n\t"                                       \
                  "v_subrev_f32_e32 %1, %3, %1\n\t"                                     \
                  "v_cmpx_gt_f32 vcc, %1, %2\n\t"                                       \
@@ -109,7 +108,7 @@ int main() {
     if (hipMalloc(&out, static_cast<size_t>(blocks) * kBlock * sizeof(float)) != hipSuccess)
         return 1;
     int rc = run<0>(out, blocks, 20000, simds, 8 * 9, "walk step (9 VALU + 1 SALU)");
-    rc |= run<3>(out, blocks, 20000, simds, 8 * 8, "walk step, short divide (8 VALU + 1 SALU)");
+    rc |= run<3>(out, blocks, 20000, simds, 8 * 7, "walk step, short divide (7 VALU + 1 SALU)");
     rc |= run<1>(out, blocks, 20000, simds, 8, "independent v_fma_f32");
     rc |= run<2>(out, blocks, 20000, simds, 8, "independent v_add_f32");
     (void)hipFree(out);
