@@ -145,9 +145,9 @@ tvs_status tvs_compute_bands(tvs_ctx *ctx, uint32_t first_sector,
 /* The batched walk's divide.  Regular steps normally use a four-operation
  * correctly rounded divide (split reciprocal + exact residual step).  With
  * the short divide on (the default), a sector whose every step gives the
- * same quotient with a three-operation divide, for every height difference
+ * same quotient with a two-operation divide, for every height difference
  * the uploaded DEM can produce (checked on the device before the first run
- * after an upload or band change), uses the three-operation one; results
+ * after an upload or band change), uses the two-operation one; results
  * are identical either way.  `verified_sectors` counts the sectors that
  * passed the last checks.  No reference counterpart (kernel.rs:175 is a
  * plain f32 divide). */

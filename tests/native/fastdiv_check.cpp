@@ -130,6 +130,16 @@ uint64_t fastdiv_midpoints(uint64_t want, uint64_t seed, uint64_t *found) {
     return bad;
 }
 
+// One pair through all three divides: out[0] = IEEE, out[1] = split
+// reciprocal, out[2] = short divide.
+void fastdiv_pair(uint32_t a_bits, uint32_t d_bits, float *out) {
+    const float a = bits(a_bits), d = bits(d_bits);
+    const tvs::Reciprocal y = tvs::split_reciprocal(d);
+    out[0] = a / d;
+    out[1] = tvs::div_by_reciprocal(a, d, y.hi, y.lo);
+    out[2] = tvs::div_short(a, y.hi, y.lo);
+}
+
 // Hard divisors: significands near 1 and near 2 (all ones), powers of two.
 uint64_t fastdiv_edges(uint64_t n, uint64_t seed) {
     uint64_t bad = 0, s = seed;

@@ -29,6 +29,8 @@ def fd(tmp_path_factory):
     lib.fastdiv_midpoints.restype = ctypes.c_uint64
     lib.fastdiv_midpoints.argtypes = [ctypes.c_uint64, ctypes.c_uint64,
                                       ctypes.POINTER(ctypes.c_uint64)]
+    lib.fastdiv_pair.restype = None
+    lib.fastdiv_pair.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]
     lib.fastdiv_heights.restype = ctypes.c_uint64
     lib.fastdiv_heights.argtypes = [ctypes.c_int32] * 5 + [ctypes.c_float, ctypes.c_void_p,
                                                           ctypes.c_uint64]
@@ -91,3 +93,16 @@ def test_workload_domain(fd, oracle, w, scale, los, obs, hmax):
     d = np.unique(dists[dists > 0]).astype(np.float32)
     bad = fd.fastdiv_heights(0, hmax, 0, hmax, 97, obs, d.ctypes.data, d.size)
     assert bad == 0
+
+
+@pytest.mark.parametrize("a_bits,d_bits", [(0xc4b6f4d0, 0x42fcf556), (0xc77240a6, 0x437bc406)])
+def test_short_divide_can_be_wrong(fd, a_bits, d_bits):
+    """The two-operation short divide misses the correctly rounded quotient
+    for some pairs (found among the height differences of a full int16 DEM
+    and the band distances of the 1201 x 1201 workload), which is why the
+    walk uses it only for sectors the device has verified; the split-
+    reciprocal divide is right on them."""
+    out = np.zeros(3, np.float32)
+    fd.fastdiv_pair(a_bits, d_bits, out.ctypes.data)
+    assert out[1].view(np.uint32) == out[0].view(np.uint32)
+    assert out[2].view(np.uint32) != out[0].view(np.uint32)

@@ -7,7 +7,8 @@
 // instruction per CU for a 16-bit typed load (the one-point walk's load), a
 // 16-bit raw load, a 4-byte-aligned dword (two heights, the two-point walk's
 // load when aligned), a dword at 2 mod 4 (misaligned), a two-component
-// typed load, and a dwordx2 (two f32 heights) aligned and at 4 mod 8.
+// typed load, and a dwordx2 (two f32 or four int16 heights) at each
+// 2-byte alignment.
 //
 // Build: make -C total-viewsheds_amd probe.  Run: total-viewsheds_amd/load_probe
 #include <hip/hip_runtime.h>
@@ -58,8 +59,9 @@ __global__ __launch_bounds__(kBlock) void load_kernel(const int16_t *buf, float 
                 const tvs_probe_float2 t = tvs_probe_load_format_xy(r, lane * 4, so, 0);
                 acc += t.x + t.y;
             }
-            if (MODE == 5 || MODE == 6) {   // dwordx2 (two f32 heights), 8-aligned / 4 mod 8
-                const auto t = __builtin_amdgcn_raw_buffer_load_b64(r, lane * 8, so + (MODE == 6 ? 4u : 0u), 0);
+            if (MODE >= 5) {   // dwordx2: 8-aligned, 4 mod 8, 2 mod 8, 6 mod 8
+                const uint32_t mis = MODE == 6 ? 4u : MODE == 7 ? 2u : MODE == 8 ? 6u : 0u;
+                const auto t = __builtin_amdgcn_raw_buffer_load_b64(r, lane * 8, so + mis, 0);
                 acc += __uint_as_float((t[0] ^ t[1]) & 0x3fffffffu);
             }
         }
@@ -109,6 +111,8 @@ int main() {
     rc |= run<4>(buf, out, blocks, cus, 2000, "typed 16_16 (format_xy, two heights)");
     rc |= run<5>(buf, out, blocks, cus, 2000, "dwordx2, 8-byte aligned (two f32 heights)");
     rc |= run<6>(buf, out, blocks, cus, 2000, "dwordx2 at 4 mod 8 (two f32 heights)");
+    rc |= run<7>(buf, out, blocks, cus, 2000, "dwordx2 at 2 mod 8 (four int16 heights)");
+    rc |= run<8>(buf, out, blocks, cus, 2000, "dwordx2 at 6 mod 8 (four int16 heights)");
     (void)hipFree(buf);
     (void)hipFree(out);
     return rc;
