@@ -365,8 +365,8 @@ def main() -> int:
         a = args.summarize_pmc
         return summarize_pmc(a[0], a[1], a[2], a[3], int(a[4]), a[5])
     if args.gpus != 1:
-        print("bench.py measures one GPU: the engine runs on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        print("bench.py measures one GPU: the engine runs on a single MI355X; "
+              "sharding over several GPUs is not built", file=sys.stderr)
         return 2
 
     import tvs_amd

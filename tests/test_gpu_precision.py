@@ -2,8 +2,9 @@
 angles, f64 per-point sums over the 180 sectors, compared with a full-FP64
 walk of the oracle (same band data; height difference, angle, max-scan and
 surface sum in double).  Reports the visibility flips of the f32 walk and
-the TVS relative-error distribution (max, p99) and writes them to
-gpurun_out/precision_main-1201.json when that folder exists."""
+the TVS relative-error distribution (max, p99), and writes them to
+precision_main-1201.json in the folder named by TVS_REPORT_DIR when that
+variable is set."""
 import json
 import os
 import sys
@@ -17,7 +18,6 @@ from tvs_amd import demgen
 pytestmark = pytest.mark.gpu
 
 THREADS = min(16, os.cpu_count() or 1)
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_precision_report_main_1201(oracle):
@@ -54,8 +54,8 @@ def test_precision_report_main_1201(oracle):
         "points": int(rel32.size),
     }
     print(json.dumps(report), flush=True)
-    out = os.path.join(ROOT, "gpurun_out")
-    if os.path.isdir(out):
+    out = os.environ.get("TVS_REPORT_DIR", "")
+    if out and os.path.isdir(out):
         with open(os.path.join(out, "precision_main-1201.json"), "w") as f:
             json.dump(report, f, indent=1)
     sys.stdout.flush()
