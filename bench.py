@@ -244,8 +244,18 @@ def measure_next_rows(wl) -> dict:
             res["ring_fixed_slot_bytes"] = int(4 * bands * tvs_amd.ring_count_per_band(
                 wl.rings_per_km, wl.max_line_of_sight))
             e.run(0, 180, "batched")
+            e.sync()
+            # Host to host, PCIe included: upload the DEM, band data for 180
+            # sectors, the walk, surfaces back (warm engine and shape).
+            t0 = time.perf_counter()
+            e.upload(wl.elevations)
+            e.compute_bands(0, 180)
+            e.reset_surfaces()
+            e.run(0, 180, "batched")
+            surf = e.surfaces()
+            res["host_to_host_ms"] = (time.perf_counter() - t0) * 1000.0
             T = e.geometry.tvs_width
-            surf = e.surfaces().reshape(T, T)
+            surf = surf.reshape(T, T)
             png = os.path.join(tmp, "heatmap.png")
             t0 = time.perf_counter()
             bt.heatmap_png(png, surf)
@@ -265,6 +275,7 @@ def measure_next_rows(wl) -> dict:
         f"cache save {res['cache_save_ms_180']:.1f} ms / load {res['cache_load_ms_180']:.1f} ms "
         f"(180 sectors), ring export {res['ring_export_ms_one_sector']:.1f} ms "
         f"({res['ring_events_one_sector']} events), PNG {res['png_ms']:.1f} ms, "
+        f"host to host {res['host_to_host_ms']:.1f} ms, "
         f"CLI end to end {res['cli_end_to_end_ms']:.0f} ms")
     return res
 
