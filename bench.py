@@ -192,7 +192,7 @@ def measure_tiles(steps: int, warmup: int) -> dict:
                                                  "two_points")}}
 
 
-def measure_next_rows(wl) -> dict:
+def measure_next_rows(wl, shape: dict) -> dict:
     """The paths beside the walk, on the same workload: `.bt` write/read and
     the CLI end to end (input file to heatmap), the band-data cache for all
     180 sectors, one sector's compact ring export, the heatmap PNG.  Each
@@ -218,6 +218,9 @@ def measure_next_rows(wl) -> dict:
                             wl.observer_height, 0) as e:
             e.upload(wl.elevations)
             e.compute_bands(0, 180)
+            e.set_walk_shape(shape["block"], shape["unroll"], shape["xcd_contiguous"],
+                             shape["pipeline"], shape["persistent"], shape["both_directions"],
+                             shape["two_points"])
             t0 = time.perf_counter()
             for s in range(180):
                 e.cache_save(state, s)
@@ -534,7 +537,7 @@ def main() -> int:
         "bands_ms": bands_ms,
     }
     if not args.no_extra:
-        out["next_rows"] = measure_next_rows(wl)
+        out["next_rows"] = measure_next_rows(wl, shape)
         out["extra"] = [measure_large(max(2, args.steps // 2), 1, args.cpu_large),
                         measure_tiles(max(2, args.steps // 2), 1)]
     if not args.no_cpu_baseline:
