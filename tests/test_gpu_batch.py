@@ -94,7 +94,7 @@ def test_autotune_keeps_results_exact(tiles_and_truth):
         before = e.surfaces()
         best, table = e.autotune(0, 30, reps=1)
         print("autotune", best, flush=True)
-        assert len(table) == 2 * 22 + 3 and best["ms"] > 0   # persistent shapes once
+        assert len(table) == 2 * 22 + 5 and best["ms"] > 0   # persistent shapes once
         np.testing.assert_array_equal(e.surfaces(), before)   # surfaces preserved
         assert e.walk_shape()["block"] == best["block"]
         e.reset_surfaces()
@@ -115,7 +115,7 @@ def test_batch_autotune_keeps_results_exact(tiles_and_truth):
         before = [b.surfaces(i) for i in range(len(tiles))]
         best, table = b.autotune(reps=1)
         print("batch autotune", best, flush=True)
-        assert len(table) == 2 * 22 + 3 and best["ms"] > 0
+        assert len(table) == 2 * 22 + 5 and best["ms"] > 0
         for i in range(len(tiles)):
             np.testing.assert_array_equal(b.surfaces(i), before[i])
         b.reset_surfaces()

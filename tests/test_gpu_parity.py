@@ -384,7 +384,8 @@ SHAPES = [  # (block, unroll, pipeline, persistent, both directions, two points 
     (128, 4, False, False, True, False), (64, 8, False, False, False, True),
     (128, 8, False, False, False, True), (64, 4, False, False, False, True),
     (64, 4, False, False, True, True), (128, 4, False, False, True, True),
-    (64, 8, False, False, True, True)]
+    (64, 8, False, False, True, True), (64, 4, False, True, True, True),
+    (64, 8, False, True, False, True)]
 
 
 @pytest.mark.parametrize("xcd", [True, False])

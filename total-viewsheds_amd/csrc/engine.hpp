@@ -139,7 +139,8 @@ inline uint32_t walk_blocks_per_pass(uint32_t T, const WalkConfig &cfg) {
     X(128, 4, false, false, true, false) X(64, 8, false, false, false, true)             \
     X(128, 8, false, false, false, true) X(64, 4, false, false, false, true)             \
     X(64, 4, false, false, true, true) X(128, 4, false, false, true, true)               \
-    X(64, 8, false, false, true, true)
+    X(64, 8, false, false, true, true) X(64, 4, false, true, true, true)                 \
+    X(64, 8, false, true, false, true)
 
 // Scratch for the device Axes::compute (keys, ids, radix-sort ping-pong,
 // histograms).
