@@ -98,9 +98,12 @@ tvs_status prepare(tvs_batch *b) {
         tvs::TileDesc d;
         tvs_status st = tvs::prepare_tile(c, 0, tvs::kSectorSteps, 0, b->cfg, &d);
         if (st != TVS_OK) return from_tile(b, i, st);
-        // The short divide is checked against the tables the tile walks with.
+        // The short divide is checked against the tables the tile walks with,
+        // on the tile's stream; the batch walks on its own stream, so the
+        // check must be complete before the walk reads its flags.
         st = tvs::ensure_short_div(c, o->tab, 0, tvs::kSectorSteps);
         if (st != TVS_OK) return from_tile(b, i, st);
+        TVS_BHIP(b, hipStreamSynchronize(c->stream), "divide check");
         d.off = o->tab.off;
         d.pos_f = o->tab.pos_f;
         d.pos_b = o->tab.pos_b;

@@ -752,18 +752,15 @@ __global__ __launch_bounds__(kBlock) void short_div_check_kernel(StepTables tab,
     if (differ) bad[slot] = 1u;
 }
 
-// Blocks of a persistent launch: enough for eight waves per SIMD.
+// Blocks of a persistent launch on the current device: enough for eight
+// waves per SIMD.
 uint32_t persistent_blocks(uint32_t block) {
-    static uint32_t cus = 0;
-    if (!cus) {
-        int dev = 0, n = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            n <= 0)
-            n = 256;
-        cus = static_cast<uint32_t>(n);
-    }
-    return cus * (32u * 64u / block);
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        n <= 0)
+        n = 256;
+    return static_cast<uint32_t>(n) * (32u * 64u / block);
 }
 
 // `n_items` counts single-direction items; a PAIR launch has half as many
