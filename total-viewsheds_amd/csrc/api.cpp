@@ -229,6 +229,10 @@ tvs_status validate_slots(tvs_ctx *c, uint32_t first, uint32_t count) {
     return TVS_OK;
 }
 
+WalkConfig effective_walk_config(const tvs_ctx *c) {
+    return c->walk_cfg_set ? c->walk_cfg : default_walk_config(c->has_i16, true);
+}
+
 // Verifies the short divide for the unchecked slots of [first, first +
 // count) of context c against step tables `tab` (c's own, or the band
 // owner's in a batch).  No-op when the short divide is off or unavailable.
@@ -696,7 +700,8 @@ tvs_status tvs_run_sectors(tvs_ctx *c, uint32_t first, uint32_t count, tvs_metho
         c->timing.walk_launches = count;
     } else {
         // Ring capture runs the capture shape; its items must be counted for it.
-        const tvs::WalkConfig cfg = capture ? tvs::capture_walk_config(c->walk_cfg) : c->walk_cfg;
+        const tvs::WalkConfig run_cfg = tvs::effective_walk_config(c);
+        const tvs::WalkConfig cfg = capture ? tvs::capture_walk_config(run_cfg) : run_cfg;
         tvs::TileDesc d;
         if ((st = tvs::prepare_tile(c, first, count, capture, cfg, &d)) != TVS_OK)
             return st;

@@ -33,6 +33,7 @@ struct tvs_batch {
     uint32_t *d_point_begin = nullptr;
     uint32_t n_items = 0, n_points = 0;
     tvs::WalkConfig cfg;
+    bool cfg_set = false;        // false: default_walk_config for the tiles' heights
     bool prepared = false;
     bool i16 = false;
     hipGraph_t graph = nullptr;
@@ -87,6 +88,8 @@ tvs_status prepare(tvs_batch *b) {
     if (any_i16 && !all_i16)
         return bfail(b, TVS_ERR_STATE, "tiles mix int16 and f32 heights");
     b->i16 = all_i16;
+    // Mixed item costs: round-robin item order keeps the XCDs balanced.
+    if (!b->cfg_set) b->cfg = tvs::default_walk_config(b->i16, false);
     std::vector<tvs::TileDesc> descs(n);
     std::vector<uint32_t> item_begin(n + 1, 0), point_begin(n + 1, 0);
     for (size_t k = 0; k < n; ++k) {
@@ -184,7 +187,6 @@ tvs_status tvs_batch_create(const tvs_params *tiles, uint32_t n_tiles, tvs_batch
     std::stable_sort(b->order.begin(), b->order.end(), [&](uint32_t x, uint32_t y) {
         return b->tiles[x]->geo.band_deltas_size > b->tiles[y]->geo.band_deltas_size;
     });
-    b->cfg.xcd_contiguous = false;   // mixed item costs: keep the XCDs balanced
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipEventCreate(&b->ev[k]);
     if (e != hipSuccess) {
@@ -262,6 +264,7 @@ tvs_status tvs_batch_set_walk_shape(tvs_batch *b, const tvs_walk_shape *shape) {
     if (!tvs::walk_shape_supported(cfg))
         return bfail(b, TVS_ERR_INVALID_ARGUMENT, "walk shape not compiled");
     b->cfg = cfg;
+    b->cfg_set = true;
     drop_graph(b);
     return TVS_OK;
 }
@@ -274,6 +277,8 @@ tvs_status tvs_batch_autotune(tvs_batch *b, uint32_t reps, tvs_walk_shape *resul
     // (scratch), never the cumulative surfaces, so the caller's totals stay.
     TVS_BHIP(b, hipStreamSynchronize(b->stream), "sync");
     const tvs::WalkConfig keep = b->cfg;
+    const bool keep_set = b->cfg_set;
+    b->cfg_set = true;   // prepare() must use the shape under test
     tvs::WalkConfig winner = keep;
     float winner_ms = 1e30f;
     size_t k = 0;
@@ -311,6 +316,7 @@ tvs_status tvs_batch_autotune(tvs_batch *b, uint32_t reps, tvs_walk_shape *resul
         }
     }
     b->cfg = st == TVS_OK ? winner : keep;
+    b->cfg_set = st == TVS_OK ? true : keep_set;
     drop_graph(b);
     b->ran = false;   // the run events now hold tuner timings
     if (st != TVS_OK) return st;

@@ -108,6 +108,15 @@ struct WalkConfig {
     bool twin = false;        // one lane walks two adjacent points of a row
 };
 
+// The shape used until a caller sets one or runs the tuner: the fastest
+// measured on the 1201 x 1201 workload for each height type (int16: two
+// points and both directions per lane; f32: one point per lane, where two
+// would take two loads).
+inline WalkConfig default_walk_config(bool i16, bool xcd_contiguous) {
+    return i16 ? WalkConfig{64, 4, xcd_contiguous, false, false, true, true}
+               : WalkConfig{128, 16, xcd_contiguous, false, false, false, false};
+}
+
 // The one shape of ring-capturing walks (a checking path): 256 points per
 // block, 8 steps per chunk, one band per lane.
 inline WalkConfig capture_walk_config(const WalkConfig &cfg) {
@@ -184,6 +193,7 @@ struct tvs_ctx {
     std::vector<tvs::SectorMeta> meta_host;
 
     tvs::WalkConfig walk_cfg;
+    bool walk_cfg_set = false;   // false: default_walk_config for the uploaded heights
     uint32_t *d_ring_counts = nullptr;    // compact ring export
     uint64_t *d_ring_offsets = nullptr;
     uint32_t *d_ring_events = nullptr;
@@ -221,6 +231,8 @@ uint32_t band_select_capacity();
 // ---- launchers (walk.hip) ----
 hipError_t launch_walk_plain(tvs_ctx *c, uint32_t slot, uint32_t *ring_out);
 bool walk_shape_supported(const WalkConfig &cfg);
+// The shape a batched run of this context uses.
+WalkConfig effective_walk_config(const tvs_ctx *c);
 // Per slot in [first, first + count): bad[slot] = 1 if the short divide
 // differs from the split-reciprocal divide for any value of `domain` at any
 // regular step of the slot's table (bad must be zero on entry).

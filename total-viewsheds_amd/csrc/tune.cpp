@@ -46,18 +46,20 @@ tvs_status tvs_set_walk_shape(tvs_ctx *c, const tvs_walk_shape *shape) {
     if (!tvs::walk_shape_supported(cfg))
         return tfail(c, TVS_ERR_INVALID_ARGUMENT, "walk shape not compiled");
     c->walk_cfg = cfg;
+    c->walk_cfg_set = true;
     return TVS_OK;
 }
 
 tvs_status tvs_get_walk_shape(const tvs_ctx *c, tvs_walk_shape *out) {
     if (!c || !out) return TVS_ERR_INVALID_ARGUMENT;
-    out->block = c->walk_cfg.block;
-    out->unroll = c->walk_cfg.unroll;
-    out->xcd_contiguous = c->walk_cfg.xcd_contiguous ? 1u : 0u;
-    out->pipeline = c->walk_cfg.pipeline ? 1u : 0u;
-    out->persistent = c->walk_cfg.persistent ? 1u : 0u;
-    out->both_directions = c->walk_cfg.pair ? 1u : 0u;
-    out->two_points = c->walk_cfg.twin ? 1u : 0u;
+    const tvs::WalkConfig cfg = tvs::effective_walk_config(c);
+    out->block = cfg.block;
+    out->unroll = cfg.unroll;
+    out->xcd_contiguous = cfg.xcd_contiguous ? 1u : 0u;
+    out->pipeline = cfg.pipeline ? 1u : 0u;
+    out->persistent = cfg.persistent ? 1u : 0u;
+    out->both_directions = cfg.pair ? 1u : 0u;
+    out->two_points = cfg.twin ? 1u : 0u;
     out->ms = 0.0f;
     return TVS_OK;
 }
@@ -86,6 +88,8 @@ tvs_status tvs_autotune(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t rep
         return tfail(c, TVS_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
     }
     const tvs::WalkConfig keep = c->walk_cfg;
+    const bool keep_set = c->walk_cfg_set;
+    c->walk_cfg_set = true;   // each timed run uses the shape under test
     tvs::WalkConfig winner = keep;
     float winner_ms = 1e30f;
     size_t k = 0;
@@ -113,6 +117,7 @@ tvs_status tvs_autotune(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t rep
         }
     }
     c->walk_cfg = st == TVS_OK ? winner : keep;
+    c->walk_cfg_set = st == TVS_OK ? true : keep_set;
     e = hipMemcpyAsync(c->cum, saved, N * sizeof(float), hipMemcpyDeviceToDevice, c->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(c->cum64, saved64, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
