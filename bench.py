@@ -283,6 +283,32 @@ def measure_next_rows(wl, shape: dict) -> dict:
     return res
 
 
+def band_paths(wl_name: str) -> int:
+    """Band data for all 180 sectors by both device paths -- selection
+    (default) and the two full stable radix sorts per sector -- checked
+    equal, with their device times."""
+    import tvs_amd
+    wl = workload(wl_name)
+    out = {"workload": wl.name}
+    bands = {}
+    for method in ("select", "sort"):
+        with tvs_amd.Engine(wl.width, wl.spacing, wl.max_line_of_sight, wl.rings_per_km,
+                            wl.observer_height, 0) as e:
+            e.upload(wl.elevations)
+            e.set_band_method(method)
+            e.compute_bands(0, 180)
+            out[f"{method}_ms"] = e.timing().bands_ms
+            out[f"{method}_sorted_sectors"] = e.bands_sorted_count()
+            bands[method] = [e.get_band(s) for s in range(180)]
+    out["equal"] = all(np.array_equal(a[0], b[0]) and
+                       np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
+                       for a, b in zip(bands["select"], bands["sort"]))
+    log(f"band data, 180 sectors: selection {out['select_ms']:.2f} ms, "
+        f"sorts {out['sort_ms']:.1f} ms, equal {out['equal']}")
+    print(json.dumps(out), flush=True)
+    return 0 if out["equal"] else 1
+
+
 def copy_calibration(mib: int) -> int:
     """A plain device-to-device copy of known size (the runtime's copy
     kernel), to scale the FETCH_SIZE / WRITE_SIZE counters."""
@@ -366,6 +392,8 @@ def main() -> int:
     ap.add_argument("--cpu-large", action="store_true",
                     help="also time the CPU oracle on the 3601x3601 tile (a few minutes)")
     ap.add_argument("--cpu-threads", type=int, default=1)
+    ap.add_argument("--band-paths", action="store_true",
+                    help="only time band data by selection and by full sorts, checked equal")
     ap.add_argument("--copy-calibration", type=int, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration)")
     ap.add_argument("--summarize-pmc", nargs=6,
@@ -375,6 +403,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
+    if args.band_paths:
+        return band_paths(args.workload)
     if args.summarize_pmc:
         a = args.summarize_pmc
         return summarize_pmc(a[0], a[1], a[2], a[3], int(a[4]), a[5])
