@@ -113,6 +113,61 @@ def test_full_int16_range_bit_exact(oracle):
                                           err_msg=np.dtype(dtype).name)
 
 
+@pytest.mark.parametrize("w,los", [(120, 3200), (122, 3300), (121, 3300), (123, 3200)])
+def test_two_points_per_lane_even_and_odd_widths(oracle, w, los):
+    """Two points per lane pairs columns (2c, 2c + 1), shifted one column on
+    odd rows of an odd-width DEM (whose left-over column then alternates
+    between the last and the first): even and odd widths, each with an even
+    and an odd L (100 m grid: L = 32 or 33), give the oracle's surfaces."""
+    rng = np.random.default_rng(w)
+    elev = rng.integers(0, 900, size=w * w).astype(np.int16)
+    note(f"two points per lane {w}x{w}, L={los // 100}: oracle band data + walk")
+    dem = oracle.dem_new(w, 100.0, los)
+    deltas, dists = oracle.band_data_all(dem, range(180))
+    cum, _ = oracle.walk_sectors(dem, elev.astype(np.float32), 1.65, deltas, dists,
+                                 n_threads=THREADS)
+    with tvs_amd.Engine(w, 100.0, los, 5.0, 1.65, 0) as e:
+        e.upload(elev)
+        e.compute_bands(0, 180)
+        for block, unroll, pers, pair in ((64, 4, False, True), (128, 8, False, False),
+                                          (64, 8, True, False)):
+            note(f"  shape {block}x{unroll} persistent={pers} pair={pair} two points")
+            e.set_walk_shape(block, unroll, True, False, pers, pair, True)
+            e.reset_surfaces()
+            e.run(0, 180, "batched")
+            np.testing.assert_array_equal(e.surfaces().view(np.uint32), cum.view(np.uint32),
+                                          err_msg=f"{w} shape {block}x{unroll} pair={pair}")
+
+
+@pytest.mark.parametrize("w", [120, 123])
+def test_band_leaving_the_dem_is_the_oracles_panic(oracle, w):
+    """A 100 m grid with a 3 km line of sight (L = 30) on 120 or 123 columns:
+    some band of the reference's band data leaves the elevation array, where
+    kernel.rs:164 panics (the oracle's DEM-index status); the engine refuses
+    the same sectors with TVS_ERR_BAND_OUT_OF_DEM, before any walk."""
+    dem = oracle.dem_new(w, 100.0, 3000)
+    elev = np.zeros(w * w, np.int16)
+    bad = []
+    for s in range(180):
+        deltas, dists = oracle.band_data(dem, s)
+        try:
+            oracle.walk_sectors(dem, elev.astype(np.float32), 1.65, deltas[None], dists[None])
+        except oracle.OracleError as err:
+            assert "status 4" in str(err)
+            bad.append(s)
+    assert bad and len(bad) < 180
+    note(f"{w}x{w}, L=30: {len(bad)} sectors leave the DEM")
+    with tvs_amd.Engine(w, 100.0, 3000, 5.0, 1.65, 0) as e:
+        e.upload(elev)
+        for s in range(180):
+            if s in bad:
+                with pytest.raises(tvs_amd.TvsError) as err:
+                    e.compute_bands(s, 1)
+                assert err.value.status == _lib.ERR_BAND_OUT_OF_DEM, s
+            else:
+                e.compute_bands(s, 1)
+
+
 @pytest.mark.parametrize("pair", [False, True])
 def test_short_divide_verified_and_exact(small, pair):
     """The three-operation divide is verified for every sector of this DEM

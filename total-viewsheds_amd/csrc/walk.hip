@@ -577,8 +577,14 @@ __device__ __forceinline__ int divide_of(const TileDesc &d, uint32_t rel) {
 // 2*item is the first of the pair's two single-direction items (every tile
 // holds an even number of them).  With TWIN the points are taken two per
 // lane: the first twin_blocks items of a pass hold the column pairs (2c,
-// 2c + 1) of every row (T/2 pairs per row), the rest (odd T only) the last
-// column of each row, one point per lane.
+// 2c + 1) of every row (T/2 pairs per row), the rest (odd T only) the
+// column left over in each row, one point per lane.  With an odd DEM width
+// (odd T) the pairs of odd rows start one column later, (2c + 1, 2c + 2),
+// and their row's left-over column is 0 instead of T - 1: then the first
+// point of every pair has an even DEM index, so at any step the dword
+// loads of a whole wave are either all 4-byte aligned or all at 2 mod 4,
+// never mixed (a wave whose lanes span two rows would otherwise pay the
+// misaligned rate at every step).
 template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PAIR, bool TWIN>
 __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
                                           const uint32_t *__restrict__ item_begin,
@@ -600,7 +606,8 @@ __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
             const uint32_t p_raw = ob * BLOCK + threadIdx.x;
             const bool active = p_raw < n_pairs;
             const uint32_t p = active ? p_raw : n_pairs - 1u;
-            const uint32_t t0 = (p / half) * d.T + 2u * (p % half);
+            const uint32_t row = p / half;
+            const uint32_t t0 = row * d.T + 2u * (p - row * half) + (d.w & row & 1u);
             if (div == 2)
                 walk_twin<H, 2, UNROLL, PAIR>(d, rel, backward, t0, active);
             else if (div == 4)
@@ -610,13 +617,14 @@ __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
             return;
         }
     }
-    // One point per lane: consecutive points, or (TWIN) the last column.
+    // One point per lane: consecutive points, or (TWIN) each row's left-over
+    // column.
     const uint32_t lb = TWIN ? ob - (d.T * (d.T >> 1) + BLOCK - 1) / BLOCK : ob;
     const uint32_t i_raw = lb * BLOCK + threadIdx.x;
     const uint32_t n_single = TWIN ? d.T : d.N;
     const bool active = i_raw < n_single;
     const uint32_t i = active ? i_raw : n_single - 1u;   // spare lanes shadow
-    const uint32_t t = TWIN ? i * d.T + (d.T - 1u) : i;
+    const uint32_t t = TWIN ? i * d.T + ((d.w & i & 1u) ? 0u : d.T - 1u) : i;
     if (PAIR) {
         if (div == 2)
             walk_pair<H, 2, UNROLL>(d, rel, t, active);
