@@ -71,7 +71,7 @@ def check_loop(loop: str, load: str, band_steps: int) -> None:
 def test_walk_loop_shape(asm, heights, load, pipe):
     """One band per lane, 8 steps per chunk: the split-reciprocal loop spends
     9 VALU per step, the short-divide loop 7, the IEEE loop more."""
-    loops = unrolled_loops(kernel_body(asm, heights + "Lb0ELi256ELi8E" + pipe + "Lb0ELb0ELb0E"))
+    loops = unrolled_loops(kernel_body(asm, heights + "Lb0ELi256ELi8E" + pipe + "Lb0ELb0ELb0ELb0E"))
     fast = [l for l in loops if "v_div_scale" not in l]
     assert len(fast) == 2
     for loop in fast:
@@ -85,7 +85,7 @@ def test_paired_walk_loop_shape(asm):
     """Both bands of a point per lane: 16 height loads and 16 masked updates
     per 8-step chunk, one set of table reads (pos_f, pos_b + the shared
     constants), 9 (split reciprocal) or 7 (short divide) VALU per band-step."""
-    loops = unrolled_loops(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb1ELb0E"))
+    loops = unrolled_loops(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb1ELb0ELb0E"))
     fast = [l for l in loops if "v_div_scale" not in l]
     assert len(fast) == 2
     for loop in fast:
@@ -100,7 +100,7 @@ def test_two_point_walk_loop_shape(asm):
     """Two adjacent points per lane: one dword load per step fetches both
     int16 heights (8 loads for 16 band-steps), each converted by one SDWA
     instruction; 8 (short divide) or 10 VALU per band-step."""
-    loops = unrolled_loops(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb0ELb1E"))
+    loops = unrolled_loops(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb0ELb1ELb0E"))
     twin = [l for l in loops if "v_div_scale" not in l and "buffer_load_dword" in l]
     assert len(twin) == 2
     for loop in twin:

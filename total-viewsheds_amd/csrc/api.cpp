@@ -715,7 +715,7 @@ tvs_status tvs_run_sectors(tvs_ctx *c, uint32_t first, uint32_t count, tvs_metho
                 "tile descriptor");
         TVS_HIP(c, hipEventRecord(c->ev[0], c->stream), "event");
         TVS_HIP(c, tvs::launch_walk_tiles(c->stream, c->d_tile, c->d_begin, 1, n_items,
-                                          c->has_i16, capture > 0, cfg),
+                                          c->has_i16, capture > 0, d.fast == 0u, cfg),
                 "batched walk");
         TVS_HIP(c, hipEventRecord(c->ev[1], c->stream), "event");
         TVS_HIP(c, tvs::launch_reduce_tiles(c->stream, c->d_tile, c->d_begin + 2, 1, N),
@@ -783,7 +783,7 @@ tvs_status tvs_compact_rings(tvs_ctx *c, uint32_t sector, uint32_t *counts, size
     TVS_HIP(c, hipMemcpyAsync(c->d_tile, &d, sizeof d, hipMemcpyHostToDevice, c->stream), "descriptor");
     TVS_HIP(c, hipMemcpyAsync(c->d_begin, begin, sizeof begin, hipMemcpyHostToDevice, c->stream), "descriptor");
     TVS_HIP(c, tvs::launch_walk_tiles(c->stream, c->d_tile, c->d_begin, 1, n_items, c->has_i16,
-                                      true, cfg), "ring count walk");
+                                      true, d.fast == 0u, cfg), "ring count walk");
     std::vector<uint32_t> cnt(bands);
     TVS_HIP(c, hipMemcpyAsync(cnt.data(), c->d_ring_counts, bands * sizeof(uint32_t),
                               hipMemcpyDeviceToHost, c->stream), "ring counts");
@@ -813,7 +813,7 @@ tvs_status tvs_compact_rings(tvs_ctx *c, uint32_t sector, uint32_t *counts, size
     d.ring_events = c->d_ring_events;
     TVS_HIP(c, hipMemcpyAsync(c->d_tile, &d, sizeof d, hipMemcpyHostToDevice, c->stream), "descriptor");
     TVS_HIP(c, tvs::launch_walk_tiles(c->stream, c->d_tile, c->d_begin, 1, n_items, c->has_i16,
-                                      true, cfg), "ring write walk");
+                                      true, d.fast == 0u, cfg), "ring write walk");
     if (total)
         TVS_HIP(c, hipMemcpyAsync(events, c->d_ring_events, total * sizeof(uint32_t),
                                   hipMemcpyDeviceToHost, c->stream), "ring events");

@@ -36,6 +36,7 @@ struct tvs_batch {
     bool cfg_set = false;        // false: default_walk_config for the tiles' heights
     bool prepared = false;
     bool i16 = false;
+    bool ieee_only = false;      // no tile has the fast divides
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     tvs_timing timing{};
@@ -123,6 +124,8 @@ tvs_status prepare(tvs_batch *b) {
     }
     b->n_items = item_begin[n];
     b->n_points = point_begin[n];
+    b->ieee_only = std::none_of(descs.begin(), descs.end(),
+                                [](const tvs::TileDesc &d) { return d.fast != 0u; });
     (void)hipFree(b->d_tiles);
     (void)hipFree(b->d_item_begin);
     (void)hipFree(b->d_point_begin);
@@ -141,7 +144,7 @@ tvs_status prepare(tvs_batch *b) {
 tvs_status enqueue(tvs_batch *b) {
     TVS_BHIP(b, tvs::launch_walk_tiles(b->stream, b->d_tiles, b->d_item_begin,
                                        static_cast<uint32_t>(b->tiles.size()), b->n_items,
-                                       b->i16, false, b->cfg),
+                                       b->i16, false, b->ieee_only, b->cfg),
              "tile walk");
     TVS_BHIP(b, tvs::launch_reduce_tiles(b->stream, b->d_tiles, b->d_point_begin,
                                          static_cast<uint32_t>(b->tiles.size()), b->n_points),
@@ -291,7 +294,7 @@ tvs_status tvs_batch_autotune(tvs_batch *b, uint32_t reps, tvs_walk_shape *resul
         auto launch = [&]() {
             return tvs::launch_walk_tiles(b->stream, b->d_tiles, b->d_item_begin,
                                           static_cast<uint32_t>(b->tiles.size()), b->n_items,
-                                          b->i16, false, b->cfg);
+                                          b->i16, false, b->ieee_only, b->cfg);
         };
         hipError_t e = launch();   // warm-up
         if (e == hipSuccess) e = hipEventRecord(b->ev[0], b->stream);
@@ -370,7 +373,7 @@ tvs_status tvs_batch_run(tvs_batch *b, uint32_t use_graph) {
     TVS_BHIP(b, hipEventRecord(b->ev[0], b->stream), "event");
     TVS_BHIP(b, tvs::launch_walk_tiles(b->stream, b->d_tiles, b->d_item_begin,
                                        static_cast<uint32_t>(b->tiles.size()), b->n_items,
-                                       b->i16, false, b->cfg),
+                                       b->i16, false, b->ieee_only, b->cfg),
              "tile walk");
     TVS_BHIP(b, hipEventRecord(b->ev[1], b->stream), "event");
     TVS_BHIP(b, tvs::launch_reduce_tiles(b->stream, b->d_tiles, b->d_point_begin,

@@ -242,9 +242,12 @@ hipError_t launch_short_div_check(hipStream_t s, const StepTables &tab, uint32_t
 // Every compiled shape the tuners time: both item mappings, persistent
 // shapes once.
 std::vector<WalkConfig> compiled_walk_shapes();
+// ieee_only: no tile of the launch has the fast divides (TileDesc::fast
+// == 0); the kernel then unrolls the IEEE-divide loop.  Either value gives
+// the same results.
 hipError_t launch_walk_tiles(hipStream_t s, const TileDesc *tiles,
                              const uint32_t *item_begin, uint32_t n_tiles,
-                             uint32_t n_items, bool i16, bool capture,
+                             uint32_t n_items, bool i16, bool capture, bool ieee_only,
                              const WalkConfig &cfg);
 hipError_t launch_reduce_tiles(hipStream_t s, const TileDesc *tiles,
                                const uint32_t *point_begin, uint32_t n_tiles,

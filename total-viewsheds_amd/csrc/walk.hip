@@ -572,6 +572,14 @@ __device__ __forceinline__ int divide_of(const TileDesc &d, uint32_t rel) {
     return 4;
 }
 
+// Divide variants of one launch.  IEEE: every tile walks with the IEEE
+// divide (no tile passed the fast-divide checks), at the launch's unroll.
+// Otherwise the fast divides run at the launch's unroll and a tile without
+// them (batches may mix) walks with the IEEE divide one step at a time: the
+// unrolled IEEE loop holds so many temporaries that the whole kernel's
+// register count, and so the waves per SIMD of the fast loops, would follow
+// it (94 against 57 VGPRs for two points and both directions per lane).
+//
 // A work item is (tile, sector, direction, BLOCK points).  With PAIR an item
 // is (tile, sector, BLOCK points), both directions: `item` counts pairs and
 // 2*item is the first of the pair's two single-direction items (every tile
@@ -585,7 +593,8 @@ __device__ __forceinline__ int divide_of(const TileDesc &d, uint32_t rel) {
 // loads of a whole wave are either all 4-byte aligned or all at 2 mod 4,
 // never mixed (a wave whose lanes span two rows would otherwise pay the
 // misaligned rate at every step).
-template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PAIR, bool TWIN>
+template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PAIR, bool TWIN,
+          bool IEEE>
 __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
                                           const uint32_t *__restrict__ item_begin,
                                           uint32_t n_tiles, uint32_t item) {
@@ -597,7 +606,7 @@ __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
     const uint32_t ob = local - pass * d.blocks_per_pass;
     const uint32_t rel = PAIR ? pass : pass >> 1;
     const bool backward = !PAIR && (pass & 1u) != 0;
-    const int div = divide_of(d, rel);
+    const int div = IEEE ? 0 : divide_of(d, rel);
     if (TWIN) {
         const uint32_t half = d.T >> 1;
         const uint32_t n_pairs = d.T * half;
@@ -608,12 +617,14 @@ __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
             const uint32_t p = active ? p_raw : n_pairs - 1u;
             const uint32_t row = p / half;
             const uint32_t t0 = row * d.T + 2u * (p - row * half) + (d.w & row & 1u);
-            if (div == 2)
+            if (IEEE)
+                walk_twin<H, 0, UNROLL, PAIR>(d, rel, backward, t0, active);
+            else if (div == 2)
                 walk_twin<H, 2, UNROLL, PAIR>(d, rel, backward, t0, active);
             else if (div == 4)
                 walk_twin<H, 4, UNROLL, PAIR>(d, rel, backward, t0, active);
             else
-                walk_twin<H, 0, UNROLL, PAIR>(d, rel, backward, t0, active);
+                walk_twin<H, 0, 1, PAIR>(d, rel, backward, t0, active);
             return;
         }
     }
@@ -626,20 +637,24 @@ __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
     const uint32_t i = active ? i_raw : n_single - 1u;   // spare lanes shadow
     const uint32_t t = TWIN ? i * d.T + ((d.w & i & 1u) ? 0u : d.T - 1u) : i;
     if (PAIR) {
-        if (div == 2)
+        if (IEEE)
+            walk_pair<H, 0, UNROLL>(d, rel, t, active);
+        else if (div == 2)
             walk_pair<H, 2, UNROLL>(d, rel, t, active);
         else if (div == 4)
             walk_pair<H, 4, UNROLL>(d, rel, t, active);
         else
-            walk_pair<H, 0, UNROLL>(d, rel, t, active);
+            walk_pair<H, 0, 1>(d, rel, t, active);
         return;
     }
-    if (div == 2)
+    if (IEEE)
+        walk_band<H, 0, CAPTURE, UNROLL, PIPE>(d, rel, backward, t, active);
+    else if (div == 2)
         walk_band<H, 2, CAPTURE, UNROLL, PIPE>(d, rel, backward, t, active);
     else if (div == 4)
         walk_band<H, 4, CAPTURE, UNROLL, PIPE>(d, rel, backward, t, active);
     else
-        walk_band<H, 0, CAPTURE, UNROLL, PIPE>(d, rel, backward, t, active);
+        walk_band<H, 0, CAPTURE, 1, PIPE>(d, rel, backward, t, active);
 }
 
 // One work item per block, or (PERSIST) a grid of a few blocks per CU
@@ -647,20 +662,20 @@ __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
 // (dispatched to the 8 XCDs round-robin) are remapped so each XCD takes a
 // contiguous run of items: one DEM region per L2.
 template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PERSIST, bool PAIR,
-          bool TWIN>
+          bool TWIN, bool IEEE>
 __global__ __launch_bounds__(BLOCK) void walk_tiles_kernel(
     const TileDesc *__restrict__ tiles, const uint32_t *__restrict__ item_begin,
     uint32_t n_tiles, uint32_t n_items, uint32_t items_per_xcd) {
     const uint32_t b = blockIdx.x;
     if (PERSIST) {
         for (uint32_t item = b; item < n_items; item += gridDim.x)
-            walk_item<H, CAPTURE, BLOCK, UNROLL, PIPE, PAIR, TWIN>(tiles, item_begin, n_tiles,
-                                                                   item);
+            walk_item<H, CAPTURE, BLOCK, UNROLL, PIPE, PAIR, TWIN, IEEE>(tiles, item_begin,
+                                                                         n_tiles, item);
         return;
     }
     const uint32_t item = items_per_xcd ? (b & 7u) * items_per_xcd + (b >> 3) : b;
     if (item >= n_items) return;
-    walk_item<H, CAPTURE, BLOCK, UNROLL, PIPE, PAIR, TWIN>(tiles, item_begin, n_tiles, item);
+    walk_item<H, CAPTURE, BLOCK, UNROLL, PIPE, PAIR, TWIN, IEEE>(tiles, item_begin, n_tiles, item);
 }
 
 // Per point, in sector order: cum += forward; cum += backward (the f32
@@ -774,31 +789,45 @@ uint32_t persistent_blocks(uint32_t block) {
 // `n_items` counts single-direction items; a PAIR launch has half as many
 // (every tile's count is even).
 template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PERSIST, bool PAIR,
-          bool TWIN>
-hipError_t launch_tiles_t(hipStream_t s, const TileDesc *tiles, const uint32_t *item_begin,
+          bool TWIN, bool IEEE>
+hipError_t launch_tiles_v(hipStream_t s, const TileDesc *tiles, const uint32_t *item_begin,
                           uint32_t n_tiles, uint32_t n_items, bool xcd_contiguous) {
     if (PAIR && (n_items & 1u)) return hipErrorInvalidValue;
     const uint32_t items = PAIR ? n_items / 2u : n_items;
     const uint32_t per_xcd = (xcd_contiguous && !PERSIST) ? (items + 7u) / 8u : 0u;
     uint32_t grid = per_xcd ? per_xcd * 8u : items;
     if (PERSIST) grid = std::min(items, persistent_blocks(BLOCK));
-    hipLaunchKernelGGL((walk_tiles_kernel<H, CAPTURE, BLOCK, UNROLL, PIPE, PERSIST, PAIR, TWIN>),
+    hipLaunchKernelGGL((walk_tiles_kernel<H, CAPTURE, BLOCK, UNROLL, PIPE, PERSIST, PAIR, TWIN,
+                                          IEEE>),
                        dim3(grid), dim3(BLOCK), 0, s, tiles, item_begin, n_tiles, items, per_xcd);
     return hipGetLastError();
 }
 
+template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PERSIST, bool PAIR,
+          bool TWIN>
+hipError_t launch_tiles_t(hipStream_t s, const TileDesc *tiles, const uint32_t *item_begin,
+                          uint32_t n_tiles, uint32_t n_items, bool xcd_contiguous,
+                          bool ieee_only) {
+    return ieee_only
+               ? launch_tiles_v<H, CAPTURE, BLOCK, UNROLL, PIPE, PERSIST, PAIR, TWIN, true>(
+                     s, tiles, item_begin, n_tiles, n_items, xcd_contiguous)
+               : launch_tiles_v<H, CAPTURE, BLOCK, UNROLL, PIPE, PERSIST, PAIR, TWIN, false>(
+                     s, tiles, item_begin, n_tiles, n_items, xcd_contiguous);
+}
+
 template <typename H>
 hipError_t launch_tiles_h(hipStream_t s, const TileDesc *tiles, const uint32_t *item_begin,
-                          uint32_t n_tiles, uint32_t n_items, bool capture,
+                          uint32_t n_tiles, uint32_t n_items, bool capture, bool ieee_only,
                           const WalkConfig &cfg) {
     if (capture)   // ring capture is a checking path: one shape (capture_walk_config)
         return launch_tiles_t<H, true, 256, 8, false, false, false, false>(
-            s, tiles, item_begin, n_tiles, n_items, cfg.xcd_contiguous);
+            s, tiles, item_begin, n_tiles, n_items, cfg.xcd_contiguous, ieee_only);
 #define TVS_SHAPE(B, U, P, S, D, W)                                                        \
     if (cfg.block == B && cfg.unroll == U && cfg.pipeline == P && cfg.persistent == S &&   \
         cfg.pair == D && cfg.twin == W)                                                    \
         return launch_tiles_t<H, false, B, U, P, S, D, W>(s, tiles, item_begin, n_tiles,   \
-                                                          n_items, cfg.xcd_contiguous);
+                                                          n_items, cfg.xcd_contiguous,     \
+                                                          ieee_only);
     TVS_WALK_SHAPES(TVS_SHAPE)
 #undef TVS_SHAPE
     return hipErrorInvalidValue;
@@ -831,10 +860,12 @@ bool walk_shape_supported(const WalkConfig &cfg) {
 
 hipError_t launch_walk_tiles(hipStream_t s, const TileDesc *tiles, const uint32_t *item_begin,
                              uint32_t n_tiles, uint32_t n_items, bool i16, bool capture,
-                             const WalkConfig &cfg) {
+                             bool ieee_only, const WalkConfig &cfg) {
     if (n_items == 0) return hipSuccess;
-    return i16 ? launch_tiles_h<int16_t>(s, tiles, item_begin, n_tiles, n_items, capture, cfg)
-               : launch_tiles_h<float>(s, tiles, item_begin, n_tiles, n_items, capture, cfg);
+    return i16 ? launch_tiles_h<int16_t>(s, tiles, item_begin, n_tiles, n_items, capture,
+                                         ieee_only, cfg)
+               : launch_tiles_h<float>(s, tiles, item_begin, n_tiles, n_items, capture,
+                                       ieee_only, cfg);
 }
 
 hipError_t launch_short_div_check(hipStream_t s, const StepTables &tab, uint32_t first,
