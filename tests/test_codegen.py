@@ -114,6 +114,39 @@ def test_two_point_walk_loop_shape(asm):
     assert valu(short) <= 8 * 16 + 8 and valu(split) <= 10 * 16 + 8, (valu(short), valu(split))
 
 
+def kernel_info(asm: str, mangled_part: str) -> dict:
+    """The compiler's register and occupancy summary of one kernel."""
+    m = re.search(r"^(_Z\S*walk_tiles_kernel" + mangled_part + r"\S*):", asm, re.M)
+    assert m, f"kernel {mangled_part} not found"
+    start = asm.index("; Kernel info:", m.end())
+    block = asm[start:asm.index("; Occupancy:", start) + 40]
+    return {k: int(v) for k, v in re.findall(r"; (\w+): (\d+)", block)}
+
+
+@pytest.mark.parametrize("shape", ["Li64ELi4ELb0ELb0ELb1ELb1E", "Li64ELi8ELb0ELb0ELb1ELb1E",
+                                   "Li128ELi4ELb0ELb0ELb1ELb1E", "Li64ELi8ELb0ELb0ELb0ELb1E",
+                                   "Li256ELi8ELb0ELb0ELb0ELb0E"])
+def test_fast_divide_kernels_keep_seven_waves(asm, shape):
+    """The fast-divide kernels walk a tile without the fast divides one IEEE
+    step at a time, so their registers are those of the fast loops: at most
+    64 VGPRs, seven or more waves per SIMD (the unrolled IEEE loop alone
+    needs ~94 VGPRs, five waves)."""
+    info = kernel_info(asm, "IsLb0E" + shape + "Lb0E")
+    assert info["TotalNumVgprs"] <= 64 and info["Occupancy"] >= 7, info
+    assert info["ScratchSize"] == 0
+
+
+def test_ieee_only_kernel_unrolls_the_ieee_loop(asm):
+    """The IEEE-only instantiation holds the unrolled IEEE loop (16 loads of
+    two heights, IEEE divide sequences) and no fast-divide loop; the
+    fast-divide instantiation holds both fast loops and no unrolled IEEE loop."""
+    ieee = unrolled_loops(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb0ELb1ELb1E"))
+    assert ieee and all("v_div_scale" in l for l in ieee)
+    fast = unrolled_loops(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb0ELb1ELb0E"))
+    assert not any("v_div_scale" in l for l in fast)
+    assert len([l for l in fast if "buffer_load_dword" in l]) == 2
+
+
 def test_no_packed_fp32_and_no_spills(asm):
     assert not re.search(r"\bv_pk_(add|mul|fma)_f32\b", asm)
     sizes = re.findall(r"\.private_segment_fixed_size:\s+(\d+)", asm)
