@@ -146,10 +146,25 @@ def measure_large(steps: int, warmup: int, cpu: bool = False) -> dict:
     return res
 
 
-def measure_tiles(steps: int, warmup: int) -> dict:
+def cpu_tiles(tiles, sectors: int, threads: int) -> dict:
+    """The oracle walk over `sectors` sectors of every tile, summed."""
+    parts = [cpu_baseline(t, sectors, threads) for t in tiles]
+    walk_s = sum(p["walk_s"] for p in parts)
+    steps = sum(p["value"] * p["walk_s"] for p in parts)
+    return {"value": steps / walk_s, "unit": "point-steps/s", "cores": threads, "kind": "port",
+            "bands_s": sum(p["bands_s"] for p in parts), "walk_s": walk_s,
+            "walk_s_180_sectors": sum(p["walk_s_180_sectors"] for p in parts),
+            "sample": (f"{'all 180 sectors' if sectors >= 180 else f'{sectors} of 180 sectors (every {180 // sectors}th)'}"
+                       f" of each of the {len(tiles)} tiles of tiles-64, walk only (every "
+                       f"step, zero tail included), oracle C restatement of the reference CPU "
+                       f"loop, {threads} thread(s), {walk_s:.1f} s; band data timed separately")}
+
+
+def measure_tiles(steps: int, warmup: int, cpu: bool = False) -> dict:
     """64 uneven tiles (sides from {257, 401, 601, 801, 1201}, 30/90 m), all
     180 sectors of all tiles in one walk launch + one reduction, replayed
-    from a hipGraph."""
+    from a hipGraph.  With `cpu`, also the oracle walk on one core over
+    every 20th sector of every tile and on every host core over all 180."""
     import tvs_amd
     from tvs_amd import demgen
     tiles = demgen.tiles_64()
@@ -179,7 +194,7 @@ def measure_tiles(steps: int, warmup: int) -> dict:
         b.run(False)                         # the launch pair without the graph
         tm = b.timing()
     log(f"tiles-64: {ms:.2f} ms/step, {ps / ms * 1e3:.3e} executed point-steps/s")
-    return {"workload": "tiles-64", "value": ps / (ms / 1000.0), "unit": "point-steps/s",
+    res = {"workload": "tiles-64", "value": ps / (ms / 1000.0), "unit": "point-steps/s",
             "nominal_value": nominal / (ms / 1000.0),
             "ms_per_step": ms, "graph_ms": float(np.mean(graph_ms)),
             "kernel_ms": tm.walk_ms, "reduce_ms": tm.reduce_ms,
@@ -190,6 +205,13 @@ def measure_tiles(steps: int, warmup: int) -> dict:
             "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous",
                                                  "pipeline", "persistent", "both_directions",
                                                  "two_points")}}
+    if cpu:
+        log("tiles-64: CPU oracle walk, 1 thread, every 20th sector of every tile")
+        res["cpu_one_core"] = cpu_tiles(tiles, 9, 1)
+        cores = min(16, len(os.sched_getaffinity(0)))
+        log(f"tiles-64: CPU oracle walk, {cores} threads, all 180 sectors of every tile")
+        res["cpu_all_cores"] = cpu_tiles(tiles, 180, cores)
+    return res
 
 
 def measure_next_rows(wl, shape: dict) -> dict:
@@ -391,6 +413,8 @@ def main() -> int:
     ap.add_argument("--cpu-sectors", type=int, default=36)
     ap.add_argument("--cpu-large", action="store_true",
                     help="also time the CPU oracle on the 3601x3601 tile (a few minutes)")
+    ap.add_argument("--cpu-tiles", action="store_true",
+                    help="also time the CPU oracle on the 64 tiles (about a minute)")
     ap.add_argument("--cpu-threads", type=int, default=1)
     ap.add_argument("--band-paths", action="store_true",
                     help="only time band data by selection and by full sorts, checked equal")
@@ -569,7 +593,7 @@ def main() -> int:
     if not args.no_extra:
         out["next_rows"] = measure_next_rows(wl, shape)
         out["extra"] = [measure_large(max(2, args.steps // 2), 1, args.cpu_large),
-                        measure_tiles(max(2, args.steps // 2), 1)]
+                        measure_tiles(max(2, args.steps // 2), 1, args.cpu_tiles)]
     if not args.no_cpu_baseline:
         log(f"CPU baseline: oracle walk over {args.cpu_sectors} sectors, "
             f"{args.cpu_threads} thread(s)")
