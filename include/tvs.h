@@ -154,6 +154,13 @@ tvs_status tvs_compute_bands(tvs_ctx *ctx, uint32_t first_sector,
 tvs_status tvs_set_short_divide(tvs_ctx *ctx, uint32_t enable);
 tvs_status tvs_get_short_divide(tvs_ctx *ctx, uint32_t *enabled,
                                 uint32_t *verified_sectors);
+/* `holds` = 1 when the uploaded heights admit the four-operation divide:
+ * integer heights whose differences from every observer height stay within
+ * the proof's range, or non-integer heights that are all 0 or of magnitude
+ * in [2^-17, 2^38], observer heights (height + observer) likewise.  0: the
+ * batched walk divides with the IEEE divide (same results, slower).  The
+ * short divide applies to integer heights only.  No reference counterpart. */
+tvs_status tvs_get_fast_divide(const tvs_ctx *ctx, uint32_t *holds);
 
 /* How tvs_compute_bands finds each sector's window and its sight order.
  * SELECT (default): per-row binary searches over the monotone rows of the

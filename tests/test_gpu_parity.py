@@ -113,6 +113,40 @@ def test_full_int16_range_bit_exact(oracle):
                                           err_msg=np.dtype(dtype).name)
 
 
+@pytest.mark.parametrize("case", ["fraction", "tiny", "integer"])
+def test_fractional_heights_fast_divide(oracle, case):
+    """Non-integer f32 heights walk with the four-operation divide when every
+    height and observer height is 0 or of magnitude >= 2^-17 (then every
+    non-zero height difference is >= 2^-40, inside the divide's proof); a
+    DEM holding a smaller non-zero height falls back to the IEEE divide.
+    Both give the oracle's surfaces bit for bit, as do integer heights."""
+    w, spacing, los = 101, 100.0, 3300
+    rng = np.random.default_rng(21)
+    elev = (rng.integers(0, 1200, size=w * w) +
+            (rng.random(w * w) if case != "integer" else 0.0)).astype(np.float32)
+    elev[::13] = 0.0
+    if case != "integer":
+        elev[5::17] = np.float32(2.0 ** -17)
+    if case == "tiny":
+        elev[w * 50 + 50] = np.float32(1e-30)
+    note(f"{case} f32 heights 101x101: oracle band data + walk")
+    dem = oracle.dem_new(w, spacing, los)
+    deltas, dists = oracle.band_data_all(dem, range(180))
+    cum, _ = oracle.walk_sectors(dem, elev, 1.65, deltas, dists, n_threads=THREADS)
+    with tvs_amd.Engine(w, spacing, los, 5.0, 1.65, 0) as e:
+        e.upload(elev)
+        assert e.fast_divide() == (case != "tiny")
+        e.compute_bands(0, 180)
+        for shape in ((128, 16, True, False, False, False, False),
+                      (128, 8, True, False, False, True, False)):
+            e.set_walk_shape(*shape)
+            e.reset_surfaces()
+            e.run(0, 180, "batched")
+            np.testing.assert_array_equal(e.surfaces().view(np.uint32), cum.view(np.uint32),
+                                          err_msg=f"{case} {shape}")
+        assert e.short_divide()[1] == (180 if case == "integer" else 0)
+
+
 @pytest.mark.parametrize("w,los", [(120, 3200), (122, 3300), (121, 3300), (123, 3200)])
 def test_two_points_per_lane_even_and_odd_widths(oracle, w, los):
     """Two points per lane pairs columns (2c, 2c + 1), shifted one column on

@@ -191,16 +191,46 @@ std::vector<float> height_differences(int64_t lo, int64_t hi, float obs) {
 // split-reciprocal divide is used everywhere.
 constexpr size_t kMaxDivDomain = size_t(1) << 22;
 
-tvs_status upload_common(tvs_ctx *c, bool integral, int64_t lo, int64_t hi) {
+// Non-integer heights: the four-operation divide's proof needs every
+// non-zero |dh| in [2^-40, 2^40], dh = fl32(h - h0) with h a DEM value and
+// h0 = fl32(hp + obs).  If every h and every h0 is 0 or has a magnitude in
+// [2^-17, 2^38], both are multiples of 2^-40 (an f32 of magnitude >= 2^-17
+// has an ulp >= 2^-40), so a non-zero difference, and its rounding, is at
+// least 2^-40 in magnitude, and |dh| <= 2^39.  NaN and infinities fail.
+bool spaced_height(float x) {
+    const float a = std::fabs(x);
+    return x == 0.0f || (a >= 0x1p-17f && a <= 0x1p38f);
+}
+
+bool fractional_divide_holds(const float *elev, size_t count, float obs) {
+    for (size_t i = 0; i < count; ++i)
+        if (!spaced_height(elev[i]) || !spaced_height(elev[i] + obs)) return false;
+    return true;
+}
+
+// `integral`: heights are integers in [lo, hi] (the short divide's domain
+// can be enumerated); otherwise `fractional_ok` says whether the
+// four-operation divide holds for them.
+tvs_status upload_common(tvs_ctx *c, bool integral, bool fractional_ok, int64_t lo,
+                         int64_t hi) {
     float mn = 0.0f, mx = 0.0f;
-    c->fast_div_ok = integral &&
-                     fast_divide_holds(lo, hi, c->params.observer_height, &mn, &mx);
+    if (integral) {
+        c->fast_div_ok = fast_divide_holds(lo, hi, c->params.observer_height, &mn, &mx);
+    } else {
+        c->fast_div_ok = fractional_ok;
+        if (fractional_ok) {
+            mn = 0x1p-40f;   // bounds, not the extremes
+            mx = 0x1p39f;
+        }
+    }
     c->min_abs_dh = mn;
     c->max_abs_dh = mx;
     std::fill(c->short_div_checked.begin(), c->short_div_checked.end(), 0);
     dev_free(c->div_domain);
     c->div_domain_n = 0;
-    if (!c->fast_div_ok) return TVS_OK;
+    // The short divide is checked over every height difference the DEM can
+    // produce, enumerable only for integer heights.
+    if (!c->fast_div_ok || !integral) return TVS_OK;
     const std::vector<float> dom = height_differences(lo, hi, c->params.observer_height);
     if (dom.empty() || dom.size() > kMaxDivDomain) return TVS_OK;
     TVS_HIP(c, dev_alloc(&c->div_domain, dom.size()), "height differences");
@@ -482,7 +512,9 @@ tvs_status tvs_upload_dem_f32(tvs_ctx *c, const float *elev, size_t count) {
         lo = std::min(lo, static_cast<double>(v));
         hi = std::max(hi, static_cast<double>(v));
     }
-    return upload_common(c, integral, integral ? static_cast<int64_t>(lo) : 0,
+    const bool fractional_ok =
+        !integral && fractional_divide_holds(elev, count, c->params.observer_height);
+    return upload_common(c, integral, fractional_ok, integral ? static_cast<int64_t>(lo) : 0,
                          integral ? static_cast<int64_t>(hi) : -1);
 }
 
@@ -506,7 +538,7 @@ tvs_status tvs_upload_dem_i16(tvs_ctx *c, const int16_t *elev, size_t count) {
         lo = std::min<int64_t>(lo, elev[i]);
         hi = std::max<int64_t>(hi, elev[i]);
     }
-    return upload_common(c, true, lo, hi);
+    return upload_common(c, true, false, lo, hi);
 }
 
 tvs_status tvs_compute_bands(tvs_ctx *c, uint32_t first, uint32_t count) {
@@ -552,6 +584,12 @@ tvs_status tvs_compute_bands(tvs_ctx *c, uint32_t first, uint32_t count) {
 tvs_status tvs_set_short_divide(tvs_ctx *c, uint32_t enable) {
     if (!c) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null context");
     c->short_div = enable != 0;
+    return TVS_OK;
+}
+
+tvs_status tvs_get_fast_divide(const tvs_ctx *c, uint32_t *holds) {
+    if (!c || !holds) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null argument");
+    *holds = c->fast_div_ok ? 1u : 0u;
     return TVS_OK;
 }
 

@@ -172,6 +172,7 @@ def load() -> ctypes.CDLL:
         "tvs_set_band_method": (st, [vp, ctypes.c_int]),
         "tvs_set_short_divide": (st, [vp, ctypes.c_uint32]),
         "tvs_get_short_divide": (st, [vp, P(ctypes.c_uint32), P(ctypes.c_uint32)]),
+        "tvs_get_fast_divide": (st, [vp, P(ctypes.c_uint32)]),
         "tvs_batch_set_short_divide": (st, [vp, ctypes.c_uint32]),
         "tvs_bands_sorted_count": (st, [vp, P(ctypes.c_uint32)]),
         "tvs_set_walk_shape": (st, [vp, P(WalkShape)]),
@@ -404,6 +405,13 @@ class Engine:
         en, n = ctypes.c_uint32(0), ctypes.c_uint32(0)
         self._c(load().tvs_get_short_divide(self._h, ctypes.byref(en), ctypes.byref(n)))
         return bool(en.value), int(n.value)
+
+    def fast_divide(self) -> bool:
+        """Whether the uploaded heights admit the four-operation divide
+        (else the batched walk uses the IEEE divide)."""
+        v = ctypes.c_uint32(0)
+        self._c(load().tvs_get_fast_divide(self._h, ctypes.byref(v)))
+        return bool(v.value)
 
     def autotune(self, first: int = 0, count: int = 180, reps: int = 2):
         """Time every compiled launch shape on this DEM, keep the fastest."""
