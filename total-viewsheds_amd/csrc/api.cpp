@@ -191,7 +191,7 @@ std::vector<float> height_differences(int64_t lo, int64_t hi, float obs) {
 // split-reciprocal divide is used everywhere.
 constexpr size_t kMaxDivDomain = size_t(1) << 22;
 
-// Non-integer heights: the four-operation divide's proof needs every
+// Any f32 heights: the four-operation divide's proof needs every
 // non-zero |dh| in [2^-40, 2^40], dh = fl32(h - h0) with h a DEM value and
 // h0 = fl32(hp + obs).  If every h and every h0 is 0 or has a magnitude in
 // [2^-17, 2^38], both are multiples of 2^-40 (an f32 of magnitude >= 2^-17
@@ -202,26 +202,25 @@ bool spaced_height(float x) {
     return x == 0.0f || (a >= 0x1p-17f && a <= 0x1p38f);
 }
 
-bool fractional_divide_holds(const float *elev, size_t count, float obs) {
+bool spaced_divide_holds(const float *elev, size_t count, float obs) {
     for (size_t i = 0; i < count; ++i)
         if (!spaced_height(elev[i]) || !spaced_height(elev[i] + obs)) return false;
     return true;
 }
 
 // `integral`: heights are integers in [lo, hi] (the short divide's domain
-// can be enumerated); otherwise `fractional_ok` says whether the
-// four-operation divide holds for them.
-tvs_status upload_common(tvs_ctx *c, bool integral, bool fractional_ok, int64_t lo,
+// can be enumerated).  `spaced_ok`: every height and observer height passes
+// spaced_height, which also admits the four-operation divide (integer
+// heights whose range is too wide for fast_divide_holds' scan among them).
+tvs_status upload_common(tvs_ctx *c, bool integral, bool spaced_ok, int64_t lo,
                          int64_t hi) {
     float mn = 0.0f, mx = 0.0f;
-    if (integral) {
-        c->fast_div_ok = fast_divide_holds(lo, hi, c->params.observer_height, &mn, &mx);
-    } else {
-        c->fast_div_ok = fractional_ok;
-        if (fractional_ok) {
-            mn = 0x1p-40f;   // bounds, not the extremes
-            mx = 0x1p39f;
-        }
+    c->fast_div_ok =
+        integral && fast_divide_holds(lo, hi, c->params.observer_height, &mn, &mx);
+    if (!c->fast_div_ok && spaced_ok) {
+        c->fast_div_ok = true;
+        mn = 0x1p-40f;   // bounds, not the extremes
+        mx = 0x1p39f;
     }
     c->min_abs_dh = mn;
     c->max_abs_dh = mx;
@@ -512,9 +511,8 @@ tvs_status tvs_upload_dem_f32(tvs_ctx *c, const float *elev, size_t count) {
         lo = std::min(lo, static_cast<double>(v));
         hi = std::max(hi, static_cast<double>(v));
     }
-    const bool fractional_ok =
-        !integral && fractional_divide_holds(elev, count, c->params.observer_height);
-    return upload_common(c, integral, fractional_ok, integral ? static_cast<int64_t>(lo) : 0,
+    const bool spaced_ok = spaced_divide_holds(elev, count, c->params.observer_height);
+    return upload_common(c, integral, spaced_ok, integral ? static_cast<int64_t>(lo) : 0,
                          integral ? static_cast<int64_t>(hi) : -1);
 }
 
