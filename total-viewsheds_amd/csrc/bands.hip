@@ -226,7 +226,8 @@ __global__ __launch_bounds__(kCompileThreads) void band_compile_kernel(
 // Per sector slot: cumulative offsets (block scan), 1/d, curvature term,
 // surface weight, and the SectorMeta ranges.
 __global__ __launch_bounds__(kScanThreads) void step_table_kernel(
-    StepTables tab, uint32_t first_slot, uint32_t n_steps) {
+    StepTables tab, uint32_t first_slot, uint32_t n_steps, uint32_t copy_offset,
+    uint32_t parity) {
     __shared__ int32_t part[kScanThreads];
     __shared__ int32_t red_min[kScanThreads / 64], red_max[kScanThreads / 64];
     __shared__ uint32_t red_fast[kScanThreads / 64], red_tail[kScanThreads / 64];
@@ -290,12 +291,15 @@ __global__ __launch_bounds__(kScanThreads) void step_table_kernel(
         tail = max(tail, red_tail[wv]);
     }
     const int32_t bias = max(hi, -lo);
-    // Byte offsets of int16 elements: 2 * (bias +/- off) (the host keeps
-    // them below 2^32).
+    // Byte offsets of int16 elements: 2 * (bias +/- off), plus 2 * copy_offset
+    // (copy B) where a two-point pair's element pov +/- off is odd (pov has
+    // the pairs' parity; off and -off have one parity); the host keeps them
+    // below 2^32.
     for (uint32_t i = begin; i < end; ++i) {
         const int32_t o = tab.off[tb + i];
-        tab.pos_f[tb + i] = 2u * static_cast<uint32_t>(bias + o);
-        tab.pos_b[tb + i] = 2u * static_cast<uint32_t>(bias - o);
+        const uint32_t b = ((static_cast<uint32_t>(o) + parity) & 1u) ? 2u * copy_offset : 0u;
+        tab.pos_f[tb + i] = 2u * static_cast<uint32_t>(bias + o) + b;
+        tab.pos_b[tb + i] = 2u * static_cast<uint32_t>(bias - o) + b;
     }
     if (tid == 0) {
         SectorMeta m{};
@@ -588,7 +592,9 @@ hipError_t launch_band_compile(tvs_ctx *c, uint32_t sector) {
 hipError_t launch_step_tables(tvs_ctx *c, uint32_t first_slot, uint32_t count) {
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(step_table_kernel, dim3(count), dim3(kScanThreads), 0, c->stream,
-                       c->tab, first_slot, c->geo.band_deltas_size);
+                       c->tab, first_slot, c->geo.band_deltas_size,
+                       static_cast<uint32_t>(dem_copy_offset(c->geo.dem_size)),
+                       pair_parity(c->geo.dem_width, c->geo.max_los_as_points));
     return hipGetLastError();
 }
 

@@ -24,6 +24,20 @@ constexpr float kSectorShift = 0.001f;
 constexpr uint32_t kScratchSector = kSectorSteps;
 constexpr uint32_t kBandSlots = kSectorSteps + 1;
 
+// Device DEM layout: the n = w*w heights (copy A, elements [0, n)) and a
+// second copy of them starting at the odd element dem_copy_offset(n) (copy
+// B).  Two adjacent int16 heights at an odd element index straddle a dword
+// in copy A and fill one aligned dword in copy B; the step tables send each
+// step whose two-point loads would be misaligned in A to B.  Both the int16
+// and the f32 buffers have the layout, so the tables do not depend on the
+// uploaded height type.
+inline uint64_t dem_copy_offset(uint64_t n) { return n | 1u; }
+inline uint64_t dem_span(uint64_t n) { return dem_copy_offset(n) + n; }
+// Parity of the DEM index of every two-point pair's first point: pairs
+// start at even columns, and on odd-width DEMs odd rows start at column 1,
+// which makes the parity 0; on even-width DEMs it is that of L.
+inline uint32_t pair_parity(uint32_t w, uint32_t L) { return (w & 1u) ? 0u : (L & 1u); }
+
 // Steps of one sector's walk are split in three ranges, each a uniform
 // (whole-grid) property of the sector:
 //   [0, n_fast)            regular steps: d normal, the fast divide holds
@@ -79,6 +93,8 @@ struct TileDesc {
     unsigned long long *overflow;
     uint32_t stride, n_steps, first_slot, count;
     uint32_t N, T, L, w;
+    uint32_t span;               // elements from elev to the end of copy B
+    uint32_t copy_offset;        // first element of copy B
     float obs;
     uint32_t blocks_per_pass;    // ceil(N / block)
     uint32_t R, capture_slots;

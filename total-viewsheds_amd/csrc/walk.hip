@@ -98,9 +98,10 @@ __global__ __launch_bounds__(kBlock) void walk_plain_kernel(
 
 // Heights are read through a buffer resource whose base sits `bias`
 // elements below the tile: the wave-uniform step offset (made non-negative
-// by the bias) is the scalar offset of the load and the lane's point is its
-// vector offset.  The resource's range covers exactly [elev - bias,
-// elev + w*w).
+// by the bias, and moved into the DEM's second copy where a two-point load
+// would be misaligned in the first) is the scalar offset of the load and
+// the lane's point is its vector offset.  The resource's range covers
+// exactly [elev - bias, elev + span): both copies.
 template <typename H>
 __device__ __forceinline__ float load_height(__amdgpu_buffer_rsrc_t rsrc,
                                              uint32_t pov_bytes,
@@ -234,7 +235,7 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
     const uint32_t bias = const_ptr(d.meta)[slot].bias;
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<H *>(elev - bias), 0,
-        static_cast<int>((bias + d.w * d.w) * sizeof(H)), kHeightRsrcFlags<H>);
+        static_cast<int>((bias + d.span) * sizeof(H)), kHeightRsrcFlags<H>);
 
     const size_t tb = static_cast<size_t>(slot) * d.stride;
     // Position of each step relative to the resource base in int16 bytes:
@@ -275,7 +276,10 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
     }
     auto event = [&](uint32_t p) {
         ++ring_id;
-        const uint32_t id = pov + (p >> 1) - bias;   // pov +/- off
+        // p / 2 is bias +/- off, plus copy_offset for a copy-B step (bias +/-
+        // off <= 2 bias < copy_offset).
+        const uint32_t e = p >> 1;
+        const uint32_t id = pov + (e > 2u * bias ? e - d.copy_offset : e) - bias;   // pov +/- off
         if (d.ring_mode == kRingRecords) {
             if (ring_id < d.R) band_ring[ring_id] = id; else ++dropped;
         } else if (d.ring_mode == kRingCompact) {
@@ -389,7 +393,7 @@ __device__ __forceinline__ void walk_pair(const TileDesc &d, uint32_t rel, uint3
     const uint32_t bias = const_ptr(d.meta)[slot].bias;
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<H *>(elev - bias), 0,
-        static_cast<int>((bias + d.w * d.w) * sizeof(H)), kHeightRsrcFlags<H>);
+        static_cast<int>((bias + d.span) * sizeof(H)), kHeightRsrcFlags<H>);
 
     const size_t tb = static_cast<size_t>(slot) * d.stride;
     const TVS_CONST uint32_t *__restrict__ pos_f = const_ptr(d.pos_f + tb);
@@ -442,7 +446,10 @@ __device__ __forceinline__ void walk_pair(const TileDesc &d, uint32_t rel, uint3
 // Heights of two adjacent elements (the step's position for a lane's point
 // t0 and for t0 + 1, which sit next to each other in the row).  int16: one
 // dword load returns both, so a band-step costs half a load instruction
-// (the texture unit's rate per load instruction bounds this walk); f32:
+// (the texture unit's rate per load instruction bounds this walk); the
+// step tables point the steps whose pair would start at an odd element
+// into the DEM's second copy, which starts at an odd element, so every such
+// dword is 4-byte aligned (a dword at 2 mod 4 costs the unit twice).  f32:
 // two loads.  `raw` is an untyped resource over the same range as the
 // heights' typed one.
 template <typename H>
@@ -484,7 +491,7 @@ __device__ __forceinline__ void walk_twin(const TileDesc &d, uint32_t rel, bool 
     const uint32_t bias = const_ptr(d.meta)[slot].bias;
     const __amdgpu_buffer_rsrc_t raw = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<H *>(elev - bias), 0,
-        static_cast<int>((bias + d.w * d.w) * sizeof(H)), kHeightRsrcFlags<float>);
+        static_cast<int>((bias + d.span) * sizeof(H)), kHeightRsrcFlags<float>);
 
     const size_t tb = static_cast<size_t>(slot) * d.stride;
     const TVS_CONST uint32_t *__restrict__ pos0 =
