@@ -293,14 +293,14 @@ tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t cap
     if (!walk_shape_supported(cfg))
         return fail(c, TVS_ERR_INVALID_ARGUMENT, "walk shape not compiled");
     // Heights are addressed with 32-bit byte offsets from a resource that
-    // starts up to (w*w - 1) / 2 elements below the DEM and covers both of
-    // its copies (step positions are at most 4 w*w int16 bytes, twice that
-    // for f32 heights; the range is about 5 w*w int16 bytes).
+    // starts up to w*w - 1 elements below the DEM (step positions are at
+    // most 4 (w*w - 1) int16 bytes, twice that for f32 heights; DEMs with a
+    // second copy are far below these limits).
     const uint64_t w2 = static_cast<uint64_t>(c->geo.dem_width) * c->geo.dem_width;
-    if (w2 > (c->has_i16 ? (1ull << 29) : (1ull << 28)))
+    if (w2 > (c->has_i16 ? (1ull << 30) : (1ull << 29)))
         return fail(c, TVS_ERR_SIZE,
                     "DEM too wide for the batched walk's 32-bit height offsets (int16 up to "
-                    "23170 points wide, f32 up to 16384); use TVS_METHOD_PLAIN");
+                    "32768 points wide, f32 up to 23170); use TVS_METHOD_PLAIN");
     if (c->partial_cap < count) {
         dev_free(c->partial);
         c->partial_cap = 0;
@@ -502,9 +502,10 @@ tvs_status tvs_upload_dem_f32(tvs_ctx *c, const float *elev, size_t count) {
     TVS_HIP(c, hipMemcpyAsync(c->elev_f32, elev, count * sizeof(float),
                               hipMemcpyHostToDevice, c->stream),
             "elevation upload");
-    TVS_HIP(c, hipMemcpyAsync(c->elev_f32 + tvs::dem_copy_offset(count), c->elev_f32,
-                              count * sizeof(float), hipMemcpyDeviceToDevice, c->stream),
-            "elevation copy");
+    if (tvs::dem_copy_offset(count))
+        TVS_HIP(c, hipMemcpyAsync(c->elev_f32 + tvs::dem_copy_offset(count), c->elev_f32,
+                                  count * sizeof(float), hipMemcpyDeviceToDevice, c->stream),
+                "elevation copy");
     TVS_HIP(c, hipStreamSynchronize(c->stream), "elevation upload");
     dev_free(c->elev_i16);
     c->has_i16 = false;
@@ -534,12 +535,14 @@ tvs_status tvs_upload_dem_i16(tvs_ctx *c, const int16_t *elev, size_t count) {
             "elevation upload");
     TVS_HIP(c, tvs::launch_widen_i16(c->stream, c->elev_i16, c->elev_f32, count),
             "elevation widen");
-    TVS_HIP(c, hipMemcpyAsync(c->elev_i16 + tvs::dem_copy_offset(count), c->elev_i16,
-                              count * sizeof(int16_t), hipMemcpyDeviceToDevice, c->stream),
-            "elevation copy");
-    TVS_HIP(c, hipMemcpyAsync(c->elev_f32 + tvs::dem_copy_offset(count), c->elev_f32,
-                              count * sizeof(float), hipMemcpyDeviceToDevice, c->stream),
-            "elevation copy");
+    if (tvs::dem_copy_offset(count)) {
+        TVS_HIP(c, hipMemcpyAsync(c->elev_i16 + tvs::dem_copy_offset(count), c->elev_i16,
+                                  count * sizeof(int16_t), hipMemcpyDeviceToDevice, c->stream),
+                "elevation copy");
+        TVS_HIP(c, hipMemcpyAsync(c->elev_f32 + tvs::dem_copy_offset(count), c->elev_f32,
+                                  count * sizeof(float), hipMemcpyDeviceToDevice, c->stream),
+                "elevation copy");
+    }
     TVS_HIP(c, hipStreamSynchronize(c->stream), "elevation upload");
     c->has_i16 = true;
     c->has_f32 = true;

@@ -292,9 +292,9 @@ __global__ __launch_bounds__(kScanThreads) void step_table_kernel(
     }
     const int32_t bias = max(hi, -lo);
     // Byte offsets of int16 elements: 2 * (bias +/- off), plus 2 * copy_offset
-    // (copy B) where a two-point pair's element pov +/- off is odd (pov has
-    // the pairs' parity; off and -off have one parity); the host keeps them
-    // below 2^32.
+    // (copy B; 0 when the DEM has none) where a two-point pair's element
+    // pov +/- off is odd (pov has the pairs' parity; off and -off have one
+    // parity); the host keeps them below 2^32.
     for (uint32_t i = begin; i < end; ++i) {
         const int32_t o = tab.off[tb + i];
         const uint32_t b = ((static_cast<uint32_t>(o) + parity) & 1u) ? 2u * copy_offset : 0u;

@@ -277,7 +277,8 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
     auto event = [&](uint32_t p) {
         ++ring_id;
         // p / 2 is bias +/- off, plus copy_offset for a copy-B step (bias +/-
-        // off <= 2 bias < copy_offset).
+        // off <= 2 bias < copy_offset; no step goes there when the offset
+        // is 0).
         const uint32_t e = p >> 1;
         const uint32_t id = pov + (e > 2u * bias ? e - d.copy_offset : e) - bias;   // pov +/- off
         if (d.ring_mode == kRingRecords) {
