@@ -122,7 +122,9 @@ tvs_status tvs_destroy(tvs_ctx *ctx);
 tvs_status tvs_get_geometry(const tvs_ctx *ctx, tvs_geometry *out);
 
 /* Elevation upload, `GPU::setup_buffers` elevations (gpu.rs:195-200), from
- * the f32 array main.rs:51-56 builds.  Copies; the caller keeps ownership. */
+ * the f32 array main.rs:51-56 builds.  Copies; the caller keeps ownership.
+ * Heights that are all int16 values (no -0) are stored as by
+ * tvs_upload_dem_i16: the same floats, read at 2 bytes per height. */
 tvs_status tvs_upload_dem_f32(tvs_ctx *ctx, const float *elevations,
                               size_t count);
 

@@ -37,6 +37,38 @@ def tiles_and_truth(oracle):
     return tiles, truth
 
 
+def test_batch_mixed_height_types(oracle, tiles_and_truth):
+    """Tiles uploaded as int16, as integer-valued f32 (stored as int16) and
+    as non-integer f32 share one launch: it walks every tile's f32 heights,
+    each tile's surfaces equal the oracle's."""
+    tiles, truth = tiles_and_truth
+    rng = np.random.default_rng(9)
+    frac = {}
+    for i in (1, 4):
+        t = tiles[i]
+        frac[i] = (t.elevations.astype(np.float64) + 0.25 + 0.5 * rng.random(t.elevations.size)
+                   ).astype(np.float32)
+        dem = oracle.dem_new(t.width, t.spacing, t.max_line_of_sight)
+        cum, _, _ = oracle.compute_run(dem, frac[i], t.observer_height, t.rings_per_km,
+                                       rings=False, n_threads=THREADS)
+        frac[i] = (frac[i], cum)
+    with tvs_amd.Batch([(t.width, t.spacing, t.max_line_of_sight, t.rings_per_km,
+                         t.observer_height) for t in tiles]) as b:
+        for i, t in enumerate(tiles):
+            if i in frac:
+                b.upload(i, frac[i][0])
+            elif i % 2:
+                b.upload(i, t.elevations.astype(np.float32))
+            else:
+                b.upload(i, t.elevations)
+        b.compute_bands()
+        b.run(False)
+        for i in range(len(tiles)):
+            want = frac[i][1] if i in frac else truth[i]
+            np.testing.assert_array_equal(b.surfaces(i).view(np.uint32), want.view(np.uint32),
+                                          err_msg=f"tile {i}")
+
+
 @pytest.mark.parametrize("use_graph", [False, True])
 def test_batch_matches_oracle(tiles_and_truth, use_graph):
     tiles, truth = tiles_and_truth

@@ -497,3 +497,44 @@ def test_every_walk_shape_bit_exact(small, xcd):
             np.testing.assert_array_equal(e.surfaces(), cum,
                                           err_msg=f"shape {block}x{unroll} pipe={pipe} "
                                                   f"pers={pers} pair={pair} two={two}")
+
+
+@pytest.fixture(scope="module", params=["spaced", "tiny"])
+def small_fractional(request, oracle):
+    """small-201 seed 0 plus a fractional part per point: f32 heights that
+    stay f32 on the device (integer-valued f32 uploads are stored as int16).
+    "tiny" also holds one height of 1e-30, which sends the DEM to the IEEE
+    divide."""
+    wl = demgen.small_201(seed=0)
+    rng = np.random.default_rng(5)
+    elev = (wl.elevations.astype(np.float64) + 0.125 + 0.75 * rng.random(wl.elevations.size)
+            ).astype(np.float32)
+    if request.param == "tiny":
+        elev[wl.width * 100 + 37] = np.float32(1e-30)
+    note(f"small-201 fractional ({request.param}): oracle band data + walk")
+    dem = oracle.dem_new(wl.width, wl.spacing, wl.max_line_of_sight)
+    deltas, dists = oracle.band_data_all(dem, range(180))
+    cum, _ = oracle.walk_sectors(dem, elev, wl.observer_height, deltas, dists,
+                                 n_threads=THREADS)
+    return wl, elev, cum, request.param == "spaced"
+
+
+@pytest.mark.parametrize("xcd", [True, False])
+def test_every_walk_shape_bit_exact_f32(small_fractional, xcd):
+    """The f32-height kernels of every compiled shape, on non-integer
+    heights (four-operation divide, or IEEE with a tiny height), give the
+    oracle's surfaces."""
+    wl, elev, cum, fast = small_fractional
+    with make_engine(wl) as e:
+        e.upload(elev)
+        assert e.fast_divide() == fast
+        e.compute_bands(0, 180)
+        for block, unroll, pipe, pers, pair, two in SHAPES:
+            note(f"shape {block}x{unroll} pipe={pipe} pers={pers} pair={pair} two={two} "
+                 f"xcd={xcd}")
+            e.set_walk_shape(block, unroll, xcd, pipe, pers, pair, two)
+            e.reset_surfaces()
+            e.run(0, 180, "batched")
+            np.testing.assert_array_equal(e.surfaces(), cum,
+                                          err_msg=f"shape {block}x{unroll} pipe={pipe} "
+                                                  f"pers={pers} pair={pair} two={two}")

@@ -498,6 +498,20 @@ tvs_status tvs_upload_dem_f32(tvs_ctx *c, const float *elev, size_t count) {
     if (count != c->geo.dem_size)
         return fail(c, TVS_ERR_SIZE, "elevation count " + std::to_string(count) +
                                          " != width^2 " + std::to_string(c->geo.dem_size));
+    // Heights that are all int16 values (bit for bit: -0 is not) are stored
+    // as int16, which the batched walk reads two per dword load; the walk
+    // widens them back to the same floats, so the results do not change.
+    bool narrow = true;
+    for (size_t i = 0; i < count && narrow; ++i) {
+        const float v = elev[i];
+        narrow = v >= -32768.0f && v <= 32767.0f && v == std::floor(v) &&
+                 !(v == 0.0f && std::signbit(v));
+    }
+    if (narrow) {
+        std::vector<int16_t> h(count);
+        for (size_t i = 0; i < count; ++i) h[i] = static_cast<int16_t>(elev[i]);
+        return tvs_upload_dem_i16(c, h.data(), count);
+    }
     if (!c->elev_f32) TVS_HIP(c, dev_alloc(&c->elev_f32, tvs::dem_span(count)), "elevations");
     TVS_HIP(c, hipMemcpyAsync(c->elev_f32, elev, count * sizeof(float),
                               hipMemcpyHostToDevice, c->stream),

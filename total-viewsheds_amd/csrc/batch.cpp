@@ -80,14 +80,13 @@ void drop_graph(tvs_batch *b) {
 tvs_status prepare(tvs_batch *b) {
     if (b->prepared) return TVS_OK;
     const size_t n = b->tiles.size();
-    bool all_i16 = true, any_i16 = false;
+    // One height type per launch: int16 when every tile has it, else f32
+    // (every tile holds its heights as f32 too).
+    bool all_i16 = true;
     for (tvs_ctx *c : b->tiles) {
         if (!c->has_f32) return bfail(b, TVS_ERR_STATE, "a tile has no elevations");
         all_i16 = all_i16 && c->has_i16;
-        any_i16 = any_i16 || c->has_i16;
     }
-    if (any_i16 && !all_i16)
-        return bfail(b, TVS_ERR_STATE, "tiles mix int16 and f32 heights");
     b->i16 = all_i16;
     // Mixed item costs: round-robin item order keeps the XCDs balanced.
     if (!b->cfg_set) b->cfg = tvs::default_walk_config(b->i16, false);
@@ -102,6 +101,12 @@ tvs_status prepare(tvs_batch *b) {
         tvs::TileDesc d;
         tvs_status st = tvs::prepare_tile(c, 0, tvs::kSectorSteps, 0, b->cfg, &d);
         if (st != TVS_OK) return from_tile(b, i, st);
+        if (!b->i16) {
+            if (static_cast<uint64_t>(d.w) * d.w > (1ull << 29))
+                return bfail(b, TVS_ERR_SIZE, "tile " + std::to_string(i) +
+                                                  " too wide for the f32 batched walk");
+            d.elev = c->elev_f32;
+        }
         // The short divide is checked against the tables the tile walks with,
         // on the tile's stream; the batch walks on its own stream, so the
         // check must be complete before the walk reads its flags.

@@ -295,7 +295,8 @@ class Engine:
 
     def upload(self, elevations: np.ndarray) -> None:
         """int16 arrays go up as int16 (the batched method reads them at 2 B
-        per step); anything else as float32."""
+        per step); anything else as float32 (which the library stores as
+        int16 when every height is an int16 value)."""
         if elevations.dtype == np.int16:
             a = np.ascontiguousarray(elevations.reshape(-1))
             self._c(load().tvs_upload_dem_i16(self._h, _ptr(a), a.size))
