@@ -130,12 +130,13 @@ struct WalkConfig {
 };
 
 // The shape used until a caller sets one or runs the tuner: the fastest
-// measured on the 1201 x 1201 workload for each height type (int16: two
-// points and both directions per lane; f32: one point per lane, where two
-// would take two loads).
+// measured on the 1201 x 1201 workload for each height type (int16: 64
+// points per item, 8 steps per chunk, two points and both directions per
+// lane; f32: 128 x 8, both directions, one point per lane, where two would
+// take two loads).
 inline WalkConfig default_walk_config(bool i16, bool xcd_contiguous) {
-    return i16 ? WalkConfig{64, 4, xcd_contiguous, false, false, true, true}
-               : WalkConfig{128, 16, xcd_contiguous, false, false, false, false};
+    return i16 ? WalkConfig{64, 8, xcd_contiguous, false, false, true, true}
+               : WalkConfig{128, 8, xcd_contiguous, false, false, true, false};
 }
 
 // The one shape of ring-capturing walks (a checking path): 256 points per
