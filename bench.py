@@ -126,14 +126,21 @@ def measure_large(steps: int, warmup: int, cpu: bool = False) -> dict:
         bands_ms = e.timing().bands_ms
         shape, _ = e.autotune(0, 180, reps=1)
         ms, kernel_ms, tm = time_method(e, "batched", steps, warmup)
+        e.compact_rings(46)                       # first call allocates
+        t0 = time.perf_counter()
+        counts, events = e.compact_rings(46)
+        ring_ms = (time.perf_counter() - t0) * 1000.0
+        assert np.all(counts % 2 == 0), "ring counts are even"
     ps = 180 * 2 * g.computable_points_count * g.real_steps
     nominal = 180 * 2 * g.computable_points_count * g.band_deltas_size
-    log(f"large: {ms:.2f} ms/step, {ps / ms * 1e3:.3e} executed point-steps/s")
+    log(f"large: {ms:.2f} ms/step, {ps / ms * 1e3:.3e} executed point-steps/s; ring export "
+        f"of one sector {ring_ms:.1f} ms ({events.size} events)")
     res = {"workload": wl.name, "value": ps / (ms / 1000.0), "unit": "point-steps/s",
            "nominal_value": nominal / (ms / 1000.0),
            "ms_per_step": ms, "kernel_ms": kernel_ms, "reduce_ms": tm.reduce_ms,
            "point_steps_per_step": ps, "nominal_point_steps_per_step": nominal,
            "bands_ms": bands_ms,
+           "ring_export_ms_one_sector": ring_ms, "ring_events_one_sector": int(events.size),
            "achieved_GBps": ps * 2 / (kernel_ms / 1000.0) / 1e9,
            "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous", "pipeline",
                                                 "persistent", "both_directions", "two_points")}}
