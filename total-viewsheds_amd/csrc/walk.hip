@@ -349,16 +349,20 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
     if (k < d.n_steps) {
         // Zero tail: delta 0 keeps the point, distance +0 gives the same
         // angle (dh/0 - 0: +inf, -inf or NaN) at every step, and the weight
-        // 0 * 0.0174533 leaves the surface unchanged.
+        // 0 * 0.0174533 leaves the surface unchanged.  Only its first step
+        // can see the point: after it the max is at least the angle (or the
+        // angle is NaN), so every later step is hidden -- one event at most
+        // at the second step, none after.
         const uint32_t o = pos[k];
         const float dh = load_height<H>(rsrc, pov_bytes, soff<H>(o)) - h0;
         const float angle = dh / dist[k] - curv[k];
-#pragma unroll 8
-        for (; k < d.n_steps; ++k) {
-            const bool v = angle > max_angle;
-            if (CAPTURE && capture && v != visible) event(o);
-            max_angle = v ? angle : max_angle;
-            visible = v;
+        const bool v = angle > max_angle;
+        if (CAPTURE && capture && v != visible) event(o);
+        max_angle = v ? angle : max_angle;
+        visible = v;
+        if (k + 1u < d.n_steps) {
+            if (CAPTURE && capture && visible) event(o);
+            visible = false;
         }
     }
     if (CAPTURE && capture) {
