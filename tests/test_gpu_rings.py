@@ -67,3 +67,60 @@ def test_compact_rings_have_no_slot_limit(oracle):
         np.testing.assert_array_equal(
             rings.to_records(big_r.geometry, counts, events, big_r.rings_per_band),
             big_r.rings(0))
+
+
+def test_compact_rings_counts_kept_until_the_state_changes(oracle):
+    """The counts of the last sector counted are kept for the next call on
+    that sector (the size query and the events call count once) and
+    dropped by an upload: after it, the export equals that of a fresh
+    engine on the same state."""
+    wl = demgen.small_201(seed=6)
+    other = (wl.elevations[::-1].copy() + 7).astype(np.int16)
+
+    def fresh(elev, sector):
+        with tvs_amd.Engine(wl.width, wl.spacing, wl.max_line_of_sight, wl.rings_per_km,
+                            wl.observer_height) as f:
+            f.upload(elev)
+            f.compute_bands(sector, 1)
+            return f.compact_rings(sector)
+
+    with tvs_amd.Engine(wl.width, wl.spacing, wl.max_line_of_sight, wl.rings_per_km,
+                        wl.observer_height) as e:
+        e.upload(wl.elevations)
+        e.compute_bands(0, 180)
+        first = e.compact_rings(12)
+        again = e.compact_rings(12)
+        for a, b in zip(first, again):
+            np.testing.assert_array_equal(a, b)
+        for a, b in zip(first, fresh(wl.elevations, 12)):
+            np.testing.assert_array_equal(a, b)
+        e.compact_rings(13)                              # another sector in between
+        for a, b in zip(e.compact_rings(12), first):
+            np.testing.assert_array_equal(a, b)
+        e.upload(other)                                  # upload: counted again
+        got = e.compact_rings(12)
+        want = fresh(other, 12)
+        assert got[1].size != first[1].size or not np.array_equal(got[1], first[1])
+        for a, b in zip(got, want):
+            np.testing.assert_array_equal(a, b)
+        counts, events = got
+        offsets = np.concatenate([[0], np.cumsum(counts.astype(np.int64) - 1)])
+        assert offsets[-1] == events.size
+
+
+def test_compact_rings_at_1201_equal_fixed_slots():
+    """1201 x 1201: 321 602 bands, 315 tiles of the device scan of the
+    offsets (two passes of its one-block tile scan); the compact export
+    rebuilds the fixed-slot records of a capture run of the same sector."""
+    wl = demgen.main_1201()
+    with tvs_amd.Engine(wl.width, wl.spacing, wl.max_line_of_sight, wl.rings_per_km,
+                        wl.observer_height, 1) as e:
+        e.upload(wl.elevations)
+        e.compute_bands(46, 1)
+        print("compact rings 1201 sector 46", flush=True)
+        counts, events = e.compact_rings(46)
+        assert counts.size == 321602 and np.all(counts % 2 == 0)
+        e.run(46, 1, "batched")
+        R = e.rings_per_band
+        np.testing.assert_array_equal(rings.to_records(e.geometry, counts, events, R),
+                                      e.rings(0))

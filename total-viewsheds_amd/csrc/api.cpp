@@ -214,6 +214,7 @@ bool spaced_divide_holds(const float *elev, size_t count, float obs) {
 // heights whose range is too wide for fast_divide_holds' scan among them).
 tvs_status upload_common(tvs_ctx *c, bool integral, bool spaced_ok, int64_t lo,
                          int64_t hi) {
+    ++c->state_gen;
     float mn = 0.0f, mx = 0.0f;
     c->fast_div_ok =
         integral && fast_divide_holds(lo, hi, c->params.observer_height, &mn, &mx);
@@ -247,6 +248,7 @@ namespace tvs {
 // Bounds check of slots [first, first+count) against this context's
 // geometry (meta_host already filled); marks them ready.
 tvs_status validate_slots(tvs_ctx *c, uint32_t first, uint32_t count) {
+    ++c->state_gen;
     for (uint32_t s = first; s < first + count; ++s) {
         const tvs_status st = check_bounds(c, s);
         if (st != TVS_OK) {
@@ -835,7 +837,9 @@ tvs_status tvs_compact_rings(tvs_ctx *c, uint32_t sector, uint32_t *counts, size
     }
     if (!c->d_ring_counts) {
         TVS_HIP(c, dev_alloc(&c->d_ring_counts, bands), "ring counts");
-        TVS_HIP(c, dev_alloc(&c->d_ring_offsets, bands), "ring offsets");
+        TVS_HIP(c, dev_alloc(&c->d_ring_offsets,
+                             tvs::ring_offsets_len(static_cast<uint32_t>(bands))),
+                "ring offsets");
     }
     const tvs::WalkConfig cfg = tvs::capture_walk_config(c->walk_cfg);
     tvs::TileDesc d;
@@ -845,21 +849,33 @@ tvs_status tvs_compact_rings(tvs_ctx *c, uint32_t sector, uint32_t *counts, size
     const uint32_t begin[4] = {0u, n_items, 0u, N};
     d.ring_mode = tvs::kRingCount;
     d.ring_counts = c->d_ring_counts;
-    TVS_HIP(c, hipMemcpyAsync(c->d_tile, &d, sizeof d, hipMemcpyHostToDevice, c->stream), "descriptor");
     TVS_HIP(c, hipMemcpyAsync(c->d_begin, begin, sizeof begin, hipMemcpyHostToDevice, c->stream), "descriptor");
-    TVS_HIP(c, tvs::launch_walk_tiles(c->stream, c->d_tile, c->d_begin, 1, n_items, c->has_i16,
-                                      true, d.fast == 0u, cfg), "ring count walk");
-    std::vector<uint32_t> cnt(bands);
-    TVS_HIP(c, hipMemcpyAsync(cnt.data(), c->d_ring_counts, bands * sizeof(uint32_t),
-                              hipMemcpyDeviceToHost, c->stream), "ring counts");
-    TVS_HIP(c, hipStreamSynchronize(c->stream), "ring counts");
-    std::vector<uint64_t> offs(bands);
-    uint64_t total = 0;
-    for (size_t k = 0; k < bands; ++k) {
-        offs[k] = total;
-        total += cnt[k] - 1u;        // events after the PoV slot
+    // Counts and offsets of this sector are kept until the next upload or
+    // band change, so the usual pair of calls (size query, then events)
+    // walks the sector's bands for counting once.
+    const bool counted = c->ring_cache_gen == c->state_gen && c->ring_cache_sector == sector;
+    if (!counted) {
+        c->ring_cache_gen = ~uint64_t(0);
+        TVS_HIP(c, hipMemcpyAsync(c->d_tile, &d, sizeof d, hipMemcpyHostToDevice, c->stream), "descriptor");
+        TVS_HIP(c, tvs::launch_walk_tiles(c->stream, c->d_tile, c->d_begin, 1, n_items,
+                                          c->has_i16, true, d.fast == 0u, cfg),
+                "ring count walk");
+        TVS_HIP(c, tvs::launch_ring_offsets(c->stream, c->d_ring_counts,
+                                            static_cast<uint32_t>(bands), c->d_ring_offsets),
+                "ring offsets");
+        TVS_HIP(c, hipMemcpyAsync(&c->ring_cache_total, c->d_ring_offsets + bands,
+                                  sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream),
+                "ring total");
     }
-    if (counts) std::copy(cnt.begin(), cnt.end(), counts);
+    if (counts)
+        TVS_HIP(c, hipMemcpyAsync(counts, c->d_ring_counts, bands * sizeof(uint32_t),
+                                  hipMemcpyDeviceToHost, c->stream), "ring counts");
+    TVS_HIP(c, hipStreamSynchronize(c->stream), "ring counts");
+    if (!counted) {
+        c->ring_cache_gen = c->state_gen;
+        c->ring_cache_sector = sector;
+    }
+    const uint64_t total = c->ring_cache_total;
     if (n_events) *n_events = total;
     if (!events) return TVS_OK;
     if (events_capacity < total)
@@ -871,8 +887,6 @@ tvs_status tvs_compact_rings(tvs_ctx *c, uint32_t sector, uint32_t *counts, size
         TVS_HIP(c, dev_alloc(&c->d_ring_events, total), "ring events");
         c->ring_events_cap = total;
     }
-    TVS_HIP(c, hipMemcpyAsync(c->d_ring_offsets, offs.data(), bands * sizeof(uint64_t),
-                              hipMemcpyHostToDevice, c->stream), "ring offsets");
     d.ring_mode = tvs::kRingCompact;
     d.ring_offsets = c->d_ring_offsets;
     d.ring_events = c->d_ring_events;

@@ -217,9 +217,15 @@ struct tvs_ctx {
     tvs::WalkConfig walk_cfg;
     bool walk_cfg_set = false;   // false: default_walk_config for the uploaded heights
     uint32_t *d_ring_counts = nullptr;    // compact ring export
-    uint64_t *d_ring_offsets = nullptr;
+    uint64_t *d_ring_offsets = nullptr;   // u64[2N + 1 + tiles]: offsets, total, scan scratch
     uint32_t *d_ring_events = nullptr;
     size_t ring_events_cap = 0;
+    // Changes with every upload and band change; the compact ring export
+    // keeps the counts and offsets of the last sector it counted under it.
+    uint64_t state_gen = 0;
+    uint64_t ring_cache_gen = ~uint64_t(0);
+    uint32_t ring_cache_sector = ~0u;
+    uint64_t ring_cache_total = 0;
     tvs::TileDesc *d_tile = nullptr;     // device copy of this DEM's descriptor
     uint32_t *d_begin = nullptr;         // {0, items} and {0, N}
     float *cum = nullptr;      // f32[N]  cumulative surfaces
@@ -275,6 +281,13 @@ hipError_t launch_reduce_tiles(hipStream_t s, const TileDesc *tiles,
                                const uint32_t *point_begin, uint32_t n_tiles,
                                uint32_t n_points);
 hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, size_t n, uint32_t v);
+// Entries of `offsets` a compact ring export of n bands needs: n offsets,
+// the total, and the scan's per-tile scratch.
+size_t ring_offsets_len(uint32_t n);
+// offsets[k] = sum over j < k of (counts[j] - 1) (each band's events after
+// its PoV slot), offsets[n] = the total.
+hipError_t launch_ring_offsets(hipStream_t s, const uint32_t *counts, uint32_t n,
+                               uint64_t *offsets);
 hipError_t launch_widen_i16(hipStream_t s, const int16_t *in, float *out,
                             size_t n);
 

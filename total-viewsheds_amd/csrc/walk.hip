@@ -787,6 +787,84 @@ __global__ __launch_bounds__(kBlock) void short_div_check_kernel(StepTables tab,
     if (differ) bad[slot] = 1u;
 }
 
+// Exclusive scan of the compact ring export's per-band event counts
+// (counts[k] - 1: the count includes the PoV slot), in three launches: per
+// tile of kScanTile bands the sum, then the tiles' exclusive scan in one
+// block (the total last), then each tile's bands.  Integer sums: the order
+// of the additions does not matter.
+constexpr uint32_t kScanBlock = 256;
+constexpr uint32_t kScanItems = 4;
+constexpr uint32_t kScanTile = kScanBlock * kScanItems;
+
+// Inclusive scan of one value per thread over the block (Hillis-Steele).
+__device__ __forceinline__ uint64_t block_inclusive_scan(uint64_t v, uint64_t *buf) {
+    const uint32_t t = threadIdx.x;
+    buf[t] = v;
+    __syncthreads();
+    for (uint32_t o = 1; o < kScanBlock; o <<= 1) {
+        const uint64_t add = t >= o ? buf[t - o] : 0u;
+        __syncthreads();
+        buf[t] += add;
+        __syncthreads();
+    }
+    const uint64_t r = buf[t];
+    __syncthreads();   // buf is reused by the caller's next scan
+    return r;
+}
+
+__global__ __launch_bounds__(kScanBlock) void ring_tile_sums_kernel(const uint32_t *counts,
+                                                                    uint32_t n,
+                                                                    uint64_t *tile_sums) {
+    __shared__ uint64_t buf[kScanBlock];
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint64_t s = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kScanItems; ++j)
+        if (base + j < n) s += counts[base + j] - 1u;
+    const uint64_t incl = block_inclusive_scan(s, buf);
+    if (threadIdx.x == kScanBlock - 1u) tile_sums[blockIdx.x] = incl;
+}
+
+__global__ __launch_bounds__(kScanBlock) void ring_tile_scan_kernel(uint64_t *tile_sums,
+                                                                    uint32_t tiles,
+                                                                    uint64_t *total) {
+    __shared__ uint64_t buf[kScanBlock];
+    uint64_t carry = 0;
+    for (uint32_t b = 0; b < tiles; b += kScanBlock) {
+        const uint32_t i = b + threadIdx.x;
+        const uint64_t v = i < tiles ? tile_sums[i] : 0u;
+        const uint64_t incl = block_inclusive_scan(v, buf);
+        if (i < tiles) tile_sums[i] = carry + incl - v;
+        // The chunk's total, from the last thread's inclusive sum.
+        if (threadIdx.x == kScanBlock - 1u) buf[0] = incl;
+        __syncthreads();
+        carry += buf[0];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(kScanBlock) void ring_offsets_kernel(const uint32_t *counts,
+                                                                  uint32_t n,
+                                                                  const uint64_t *tile_offsets,
+                                                                  uint64_t *offsets) {
+    __shared__ uint64_t buf[kScanBlock];
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint64_t v[kScanItems];
+    uint64_t s = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kScanItems; ++j) {
+        v[j] = base + j < n ? static_cast<uint64_t>(counts[base + j] - 1u) : 0u;
+        s += v[j];
+    }
+    uint64_t run = tile_offsets[blockIdx.x] + block_inclusive_scan(s, buf) - s;
+#pragma unroll
+    for (uint32_t j = 0; j < kScanItems; ++j) {
+        if (base + j < n) offsets[base + j] = run;
+        run += v[j];
+    }
+}
+
 // Blocks of a persistent launch on the current device: enough for eight
 // waves per SIMD.
 uint32_t persistent_blocks(uint32_t block) {
@@ -897,6 +975,24 @@ hipError_t launch_reduce_tiles(hipStream_t s, const TileDesc *tiles,
     if (n_points == 0) return hipSuccess;
     hipLaunchKernelGGL(reduce_tiles_kernel, dim3((n_points + kBlock - 1) / kBlock),
                        dim3(kBlock), 0, s, tiles, point_begin, n_tiles, n_points);
+    return hipGetLastError();
+}
+
+size_t ring_offsets_len(uint32_t n) {
+    return static_cast<size_t>(n) + 1u + (n + kScanTile - 1u) / kScanTile;
+}
+
+hipError_t launch_ring_offsets(hipStream_t s, const uint32_t *counts, uint32_t n,
+                               uint64_t *offsets) {
+    if (n == 0) return hipSuccess;
+    const uint32_t tiles = (n + kScanTile - 1u) / kScanTile;
+    uint64_t *tile_sums = offsets + n + 1u;
+    hipLaunchKernelGGL(ring_tile_sums_kernel, dim3(tiles), dim3(kScanBlock), 0, s, counts, n,
+                       tile_sums);
+    hipLaunchKernelGGL(ring_tile_scan_kernel, dim3(1), dim3(kScanBlock), 0, s, tile_sums, tiles,
+                       offsets + n);
+    hipLaunchKernelGGL(ring_offsets_kernel, dim3(tiles), dim3(kScanBlock), 0, s, counts, n,
+                       tile_sums, offsets);
     return hipGetLastError();
 }
 
