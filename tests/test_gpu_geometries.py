@@ -1,0 +1,93 @@
+"""Seeded random geometries through the ABI against the oracle, bit for bit:
+widths odd and even, L of either parity, four grid spacings, observer
+heights from 0 to 30 m, terrain from flat-ish to the whole int16 range,
+int16, integer-valued f32 and non-integer f32 uploads, the default and a
+drawn launch shape.  Sectors whose band leaves the DEM (the reference's
+kernel.rs:164 panic) must be refused, as the oracle refuses them."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import tvs_amd
+from tvs_amd import demgen, _lib
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+from test_gpu_parity import SHAPES  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def draw(seed: int):
+    rng = np.random.default_rng(1000 + seed)
+    w = int(rng.integers(31, 171))
+    spacing = float(rng.choice([10.0, 30.0, 90.0, 100.0]))
+    L = int(rng.integers(3, max(4, w // 3) + 1))
+    los = int(L * spacing)
+    obs = float(rng.choice([0.0, 1.65, 1.8, 30.0]))
+    kind = ["fractal", "noise", "extreme"][seed % 3]
+    if kind == "fractal":
+        elev = demgen.fractal_dem(w, spacing, 2.4, float(rng.integers(10, 4000)), seed)
+    elif kind == "noise":
+        elev = rng.integers(-200, 2500, size=(w, w)).astype(np.int16)
+    else:
+        elev = rng.integers(-32768, 32768, size=(w, w)).astype(np.int16)
+    elev = elev.reshape(-1).astype(np.int16)
+    upload = ["int16", "f32", "f32 fraction"][(seed // 3) % 3]
+    if upload == "int16":
+        heights = elev
+    elif upload == "f32":
+        heights = elev.astype(np.float32)
+    else:
+        heights = (elev.astype(np.float64) + rng.random(elev.size) * 0.9 + 0.05).astype(np.float32)
+    shape = SHAPES[int(rng.integers(len(SHAPES)))]
+    return w, spacing, los, obs, kind, upload, heights, shape
+
+
+@pytest.mark.parametrize("seed", range(18))
+def test_random_geometry_bit_exact(oracle, seed):
+    w, spacing, los, obs, kind, upload, heights, shape = draw(seed)
+    print(f"seed {seed}: {w}x{w} @ {spacing} m, los {los} m, obs {obs} m, {kind}, "
+          f"{upload}, shape {shape}", flush=True)
+    dem = oracle.dem_new(w, spacing, los)
+    deltas, dists = oracle.band_data_all(dem, range(180))
+    f32 = heights.astype(np.float32)
+    ok = []
+    cum = np.zeros(dem.computable_points_count, np.float32)
+    for s in range(180):
+        tmp = cum.copy()                           # a refused sector adds nothing
+        try:
+            oracle.walk_sectors(dem, f32, obs, deltas[s:s + 1], dists[s:s + 1],
+                                n_threads=THREADS, cumulative=tmp)
+        except oracle.OracleError as err:
+            assert "status 4" in str(err)          # the band leaves the DEM
+            continue
+        cum = tmp
+        ok.append(s)
+    # Every sector the oracle walks, in order, accumulated in `cum`.
+    with tvs_amd.Engine(w, spacing, los, 5.0, obs, 0) as e:
+        e.upload(heights)
+        for s in range(180):
+            if s in ok:
+                e.compute_bands(s, 1)
+            else:
+                with pytest.raises(tvs_amd.TvsError) as err:
+                    e.compute_bands(s, 1)
+                assert err.value.status == _lib.ERR_BAND_OUT_OF_DEM, s
+        if not ok:
+            return
+        for shp in (None, shape):
+            if shp is not None:
+                block, unroll, pipe, pers, pair, two = shp
+                e.set_walk_shape(block, unroll, True, pipe, pers, pair, two)
+            e.reset_surfaces()
+            for a in range(len(ok)):               # contiguous runs of walkable sectors
+                if a == 0 or ok[a] != ok[a - 1] + 1:
+                    b = a
+                    while b + 1 < len(ok) and ok[b + 1] == ok[b] + 1:
+                        b += 1
+                    e.run(ok[a], b - a + 1, "batched")
+            np.testing.assert_array_equal(e.surfaces().view(np.uint32), cum.view(np.uint32),
+                                          err_msg=f"seed {seed} shape {shp}")
