@@ -1,8 +1,9 @@
 """Seeded random geometries through the ABI against the oracle, bit for bit:
 widths odd and even, L of either parity, four grid spacings, observer
 heights from 0 to 30 m, terrain from flat-ish to the whole int16 range,
-int16, integer-valued f32 and non-integer f32 uploads, the default and a
-drawn launch shape.  Sectors whose band leaves the DEM (the reference's
+int16, integer-valued f32 and non-integer f32 uploads; the plain walk, the
+batched walk in the default and a drawn launch shape, and one sector's
+ring events.  Sectors whose band leaves the DEM (the reference's
 kernel.rs:164 panic) must be refused, as the oracle refuses them."""
 import os
 import sys
@@ -11,7 +12,7 @@ import numpy as np
 import pytest
 
 import tvs_amd
-from tvs_amd import demgen, _lib
+from tvs_amd import demgen, rings, _lib
 
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 from test_gpu_parity import SHAPES  # noqa: E402
@@ -78,8 +79,17 @@ def test_random_geometry_bit_exact(oracle, seed):
                 assert err.value.status == _lib.ERR_BAND_OUT_OF_DEM, s
         if not ok:
             return
-        for shp in (None, shape):
-            if shp is not None:
+        # The first walkable sector's ring events against the oracle's
+        # records (64 slots a band, compact export rebuilt to that layout).
+        s0 = ok[0]
+        counts, events = e.compact_rings(s0)
+        _, want = oracle.walk_sectors(dem, f32, obs, deltas[s0:s0 + 1], dists[s0:s0 + 1],
+                                      rings_per_band=64, rings=True, n_threads=THREADS)
+        np.testing.assert_array_equal(rings.to_records(e.geometry, counts, events, 64), want[0],
+                                      err_msg=f"seed {seed} rings sector {s0}")
+        for shp in ("plain", None, shape):
+            method = "plain" if shp == "plain" else "batched"
+            if shp not in (None, "plain"):
                 block, unroll, pipe, pers, pair, two = shp
                 e.set_walk_shape(block, unroll, True, pipe, pers, pair, two)
             e.reset_surfaces()
@@ -88,6 +98,6 @@ def test_random_geometry_bit_exact(oracle, seed):
                     b = a
                     while b + 1 < len(ok) and ok[b + 1] == ok[b] + 1:
                         b += 1
-                    e.run(ok[a], b - a + 1, "batched")
+                    e.run(ok[a], b - a + 1, method)
             np.testing.assert_array_equal(e.surfaces().view(np.uint32), cum.view(np.uint32),
                                           err_msg=f"seed {seed} shape {shp}")
