@@ -256,6 +256,13 @@ tvs_status tvs_sync(tvs_ctx *ctx);
 /* Device times of the last run and band computation. */
 tvs_status tvs_get_timing(tvs_ctx *ctx, tvs_timing *out);
 
+/* Walk times (ms, as tvs_timing.walk_ms) of the last `n` runs of
+ * tvs_run_sectors, oldest first (at most 64 are kept; `*got` says how many
+ * were written), so runs issued back to back can be timed without a wait
+ * between them.  Waits for the context's stream.  No reference counterpart
+ * (measurement). */
+tvs_status tvs_get_walk_history(tvs_ctx *ctx, float *ms, size_t n, size_t *got);
+
 /* Message for the last failing call on `ctx` (or on a NULL context). */
 const char *tvs_last_error(const tvs_ctx *ctx);
 

@@ -205,6 +205,24 @@ def test_band_leaving_the_dem_is_the_oracles_panic(oracle, w):
                 e.compute_bands(s, 1)
 
 
+def test_walk_history(small):
+    """tvs_get_walk_history: one walk time per run, oldest first, at most 64,
+    the last equal to tvs_get_timing's (events recorded at the same points)."""
+    wl, _, _, _, _ = small
+    with make_engine(wl) as e:
+        e.upload(wl.elevations)
+        e.compute_bands(0, 180)
+        assert e.walk_history(4).size == 0
+        for _ in range(3):
+            e.run(0, 180, "batched")
+        h = e.walk_history(5)
+        assert h.size == 3 and np.all(h > 0)
+        assert abs(h[-1] - e.timing().walk_ms) <= 0.1 * h[-1] + 0.01
+        for _ in range(70):
+            e.run(0, 30, "plain")
+        assert e.walk_history(100).size == 64
+
+
 @pytest.mark.parametrize("pair", [False, True])
 def test_short_divide_verified_and_exact(small, pair):
     """The three-operation divide is verified for every sector of this DEM

@@ -484,6 +484,8 @@ tvs_status tvs_destroy(tvs_ctx *c) {
     free_sort_scratch(c->sort);
     for (auto &ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
+    for (hipEvent_t h : c->walk_hist)
+        if (h) (void)hipEventDestroy(h);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return TVS_OK;
@@ -756,13 +758,29 @@ tvs_status tvs_run_sectors(tvs_ctx *c, uint32_t first, uint32_t count, tvs_metho
                            static_cast<uint64_t>(count) * 2ull * N * c->geo.band_deltas_size};
     if (count == 0 || N == 0) return TVS_OK;
     const size_t per = 2ull * N * c->rings_per_band;
+    if (c->walk_hist.empty()) {
+        c->walk_hist.assign(2u * tvs_ctx::kWalkHistory, nullptr);
+        for (hipEvent_t &h : c->walk_hist) {
+            const hipError_t e = hipEventCreate(&h);
+            if (e != hipSuccess) {
+                for (hipEvent_t &g : c->walk_hist)
+                    if (g) (void)hipEventDestroy(g);
+                c->walk_hist.clear();
+                return fail(c, TVS_ERR_HIP, std::string("walk history events: ") +
+                                                hipGetErrorString(e));
+            }
+        }
+    }
+    hipEvent_t *hist = &c->walk_hist[2u * (c->walk_runs % tvs_ctx::kWalkHistory)];
     if (method == TVS_METHOD_PLAIN) {
         TVS_HIP(c, hipEventRecord(c->ev[0], c->stream), "event");
+        TVS_HIP(c, hipEventRecord(hist[0], c->stream), "event");
         for (uint32_t r = 0; r < count; ++r) {
             uint32_t *ring = r < capture ? c->ring + per * r : nullptr;
             TVS_HIP(c, tvs::launch_walk_plain(c, first + r, ring), "plain walk");
         }
         TVS_HIP(c, hipEventRecord(c->ev[1], c->stream), "event");
+        TVS_HIP(c, hipEventRecord(hist[1], c->stream), "event");
         TVS_HIP(c, hipEventRecord(c->ev[2], c->stream), "event");
         c->timing.walk_launches = count;
     } else {
@@ -781,15 +799,18 @@ tvs_status tvs_run_sectors(tvs_ctx *c, uint32_t first, uint32_t count, tvs_metho
                                   c->stream),
                 "tile descriptor");
         TVS_HIP(c, hipEventRecord(c->ev[0], c->stream), "event");
+        TVS_HIP(c, hipEventRecord(hist[0], c->stream), "event");
         TVS_HIP(c, tvs::launch_walk_tiles(c->stream, c->d_tile, c->d_begin, 1, n_items,
                                           c->has_i16, capture > 0, d.fast == 0u, cfg),
                 "batched walk");
         TVS_HIP(c, hipEventRecord(c->ev[1], c->stream), "event");
+        TVS_HIP(c, hipEventRecord(hist[1], c->stream), "event");
         TVS_HIP(c, tvs::launch_reduce_tiles(c->stream, c->d_tile, c->d_begin + 2, 1, N),
                 "sector reduction");
         TVS_HIP(c, hipEventRecord(c->ev[2], c->stream), "event");
         c->timing.walk_launches = 1;
     }
+    ++c->walk_runs;
     c->ring_valid = capture;
     return TVS_OK;
 }
@@ -961,6 +982,20 @@ tvs_status tvs_get_timing(tvs_ctx *c, tvs_timing *out) {
         c->timing.reduce_ms = ms;
     }
     *out = c->timing;
+    return TVS_OK;
+}
+
+tvs_status tvs_get_walk_history(tvs_ctx *c, float *ms, size_t n, size_t *got) {
+    if (!c || (!ms && n)) return fail(c, TVS_ERR_INVALID_ARGUMENT, "null argument");
+    const uint64_t kept = std::min<uint64_t>(c->walk_runs, tvs_ctx::kWalkHistory);
+    const size_t k = static_cast<size_t>(std::min<uint64_t>(n, kept));
+    TVS_HIP(c, hipStreamSynchronize(c->stream), "sync");
+    for (size_t j = 0; j < k; ++j) {
+        const uint64_t run = c->walk_runs - k + j;
+        const hipEvent_t *h = &c->walk_hist[2u * (run % tvs_ctx::kWalkHistory)];
+        TVS_HIP(c, hipEventElapsedTime(&ms[j], h[0], h[1]), "event time");
+    }
+    if (got) *got = k;
     return TVS_OK;
 }
 

@@ -195,6 +195,12 @@ struct tvs_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {};
+    // Start and end of the walk of each of the last kWalkHistory runs of
+    // tvs_run_sectors (slot = run index mod kWalkHistory), created on the
+    // first run.
+    static constexpr uint32_t kWalkHistory = 64;
+    std::vector<hipEvent_t> walk_hist;
+    uint64_t walk_runs = 0;
 
     float *elev_f32 = nullptr;
     int16_t *elev_i16 = nullptr;

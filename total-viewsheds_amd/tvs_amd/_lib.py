@@ -173,6 +173,8 @@ def load() -> ctypes.CDLL:
         "tvs_set_short_divide": (st, [vp, ctypes.c_uint32]),
         "tvs_get_short_divide": (st, [vp, P(ctypes.c_uint32), P(ctypes.c_uint32)]),
         "tvs_get_fast_divide": (st, [vp, P(ctypes.c_uint32)]),
+        "tvs_get_walk_history": (st, [vp, P(ctypes.c_float), ctypes.c_size_t,
+                                      P(ctypes.c_size_t)]),
         "tvs_batch_set_short_divide": (st, [vp, ctypes.c_uint32]),
         "tvs_bands_sorted_count": (st, [vp, P(ctypes.c_uint32)]),
         "tvs_set_walk_shape": (st, [vp, P(WalkShape)]),
@@ -406,6 +408,13 @@ class Engine:
         en, n = ctypes.c_uint32(0), ctypes.c_uint32(0)
         self._c(load().tvs_get_short_divide(self._h, ctypes.byref(en), ctypes.byref(n)))
         return bool(en.value), int(n.value)
+
+    def walk_history(self, n: int) -> np.ndarray:
+        """Walk times (ms) of the last n runs (at most 64), oldest first."""
+        out = (ctypes.c_float * max(n, 1))()
+        got = ctypes.c_size_t(0)
+        self._c(load().tvs_get_walk_history(self._h, out, n, ctypes.byref(got)))
+        return np.array(out[:got.value], np.float64)
 
     def fast_divide(self) -> bool:
         """Whether the uploaded heights admit the four-operation divide
