@@ -47,7 +47,8 @@ def draw(seed: int):
     return w, spacing, los, obs, kind, upload, heights, shape
 
 
-@pytest.mark.parametrize("seed", range(18))
+# TVS_GEOMETRY_SEEDS widens the sweep (18 seeds by default).
+@pytest.mark.parametrize("seed", range(int(os.environ.get("TVS_GEOMETRY_SEEDS", "18"))))
 def test_random_geometry_bit_exact(oracle, seed):
     w, spacing, los, obs, kind, upload, heights, shape = draw(seed)
     print(f"seed {seed}: {w}x{w} @ {spacing} m, los {los} m, obs {obs} m, {kind}, "
