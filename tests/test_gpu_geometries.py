@@ -1,4 +1,4 @@
-"""Seeded random geometries through the ABI against the oracle, bit for bit:
+"""Seeded random geometries (64 by default) through the ABI against the oracle, bit for bit:
 widths odd and even, L of either parity, four grid spacings, observer
 heights from 0 to 30 m, terrain from flat-ish to the whole int16 range,
 int16, integer-valued f32 and non-integer f32 uploads; the plain walk, the
@@ -47,8 +47,8 @@ def draw(seed: int):
     return w, spacing, los, obs, kind, upload, heights, shape
 
 
-# TVS_GEOMETRY_SEEDS widens the sweep (18 seeds by default).
-@pytest.mark.parametrize("seed", range(int(os.environ.get("TVS_GEOMETRY_SEEDS", "18"))))
+# TVS_GEOMETRY_SEEDS sets the number of geometries (64 by default).
+@pytest.mark.parametrize("seed", range(int(os.environ.get("TVS_GEOMETRY_SEEDS", "64"))))
 def test_random_geometry_bit_exact(oracle, seed):
     w, spacing, los, obs, kind, upload, heights, shape = draw(seed)
     print(f"seed {seed}: {w}x{w} @ {spacing} m, los {los} m, obs {obs} m, {kind}, "
