@@ -154,3 +154,33 @@ def test_batch_autotune_keeps_results_exact(tiles_and_truth):
         b.run(True)
         for i in range(len(tiles)):
             np.testing.assert_array_equal(b.surfaces(i), truth[i], err_msg=f"tile {i}")
+
+
+@pytest.mark.parametrize("tiny_tiles", [(0, 1, 2, 3, 4, 5), (1, 4)])
+def test_batch_ieee_divide_tiles(oracle, tiles_and_truth, tiny_tiles):
+    """Tiles whose heights fail the fast-divide range check (one non-zero
+    height below 2^-17) walk with the IEEE divide: all of them (the launch
+    takes the IEEE-only kernels) or some (those tiles take the one-step IEEE
+    loop of the fast kernels).  Every tile equals the oracle."""
+    tiles, truth = tiles_and_truth
+    want = list(truth)
+    heights = [t.elevations for t in tiles]
+    for i in tiny_tiles:
+        t = tiles[i]
+        h = t.elevations.astype(np.float32) + np.float32(0.5)
+        h[t.width * (t.width // 2) + 3] = np.float32(1e-30)
+        heights[i] = h
+        dem = oracle.dem_new(t.width, t.spacing, t.max_line_of_sight)
+        want[i], _, _ = oracle.compute_run(dem, h, t.observer_height, t.rings_per_km,
+                                           rings=False, n_threads=THREADS)
+    print(f"batch with IEEE-divide tiles {tiny_tiles}", flush=True)
+    with tvs_amd.Batch([(t.width, t.spacing, t.max_line_of_sight, t.rings_per_km,
+                         t.observer_height) for t in tiles]) as b:
+        for i, h in enumerate(heights):
+            b.upload(i, h)
+        b.compute_bands()
+        b.run(True)
+        b.sync()
+        for i in range(len(tiles)):
+            np.testing.assert_array_equal(b.surfaces(i).view(np.uint32),
+                                          want[i].view(np.uint32), err_msg=f"tile {i}")
