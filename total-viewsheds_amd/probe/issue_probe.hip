@@ -1,8 +1,8 @@
 // issue_probe -- the instruction-issue ceiling of the batched walk's step.
 //
 // Runs the walk's per-step vector sequence (sub, mul, fmac, fma, fmac,
-// subrev, v_cmpx, masked add, exec restore, max: 9 VALU + 1 SALU; 7 VALU
-// with the short divide) out of registers, with no memory access, on every SIMD at full occupancy, and
+// subrev, v_cmpx, masked add, masked move, exec restore: 9 VALU + 1 SALU;
+// 7 VALU with the short divide) out of registers, with no memory access, on every SIMD at full occupancy, and
 // reports nanoseconds per wave-instruction per SIMD.  Next to it: the same
 // count of independent FMAs and of independent adds.  The walk kernel's own
 // rate (kernel time / its VALU instructions per SIMD) is compared with these
@@ -42,8 +42,8 @@ __global__ __launch_bounds__(kBlock) void issue_kernel(float *out, int iters, fl
                  "v_subrev_f32_e32 %1, %3, %1\n\t"                                     \
                  "v_cmpx_gt_f32 vcc, %1, %2\n\t"                                       \
                  "v_add_f32_e32 %0, %4, %0\n\t"                                        \
-                 "s_mov_b64 exec, %5\n\t"                                              \
-                 "v_max_f32_e32 %0, %1, %0"                                            \
+                 "v_mov_b32_e32 %0, %1\n\t"                                            \
+                 "s_mov_b64 exec, %5"                                                  \
                  : "+v"(x##j), "+v"(y##j)                                              \
                  : "v"(h0), "s"(c0), "s"(c1), "s"(lanes)                               \
                  : "vcc");
@@ -54,8 +54,8 @@ __global__ __launch_bounds__(kBlock) void issue_kernel(float *out, int iters, fl
                  "v_subrev_f32_e32 %1, %3, %1\n\t"                                     \
                  "v_cmpx_gt_f32 vcc, %1, %2\n\t"                                       \
                  "v_add_f32_e32 %0, %4, %0\n\t"                                        \
-                 "s_mov_b64 exec, %5\n\t"                                              \
-                 "v_max_f32_e32 %0, %1, %0"                                            \
+                 "v_mov_b32_e32 %0, %1\n\t"                                            \
+                 "s_mov_b64 exec, %5"                                                  \
                  : "+v"(x##j), "+v"(y##j)                                              \
                  : "v"(h0), "s"(c0), "s"(c1), "s"(lanes)                               \
                  : "vcc");
