@@ -38,9 +38,9 @@ HBM_PEAK_GBPS = 8000.0
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 # Vector instructions per real band-step of the batched walk's unrolled
 # loop: sub, the divide (4 operations, or 2 with the verified short divide),
-# sub, cmpx, add, max, plus one conversion when two points per lane share
-# a dword load (tests/test_codegen.py holds the compiled loops to these
-# counts).
+# sub, cmpx, masked add, masked move, plus one conversion when two points
+# per lane share a dword load (tests/test_codegen.py holds the compiled
+# loops to these counts).
 VALU_PER_STEP = {"split": 9, "short": 7}
 
 
