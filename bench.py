@@ -144,7 +144,7 @@ def measure_large(steps: int, warmup: int, cpu: bool = False) -> dict:
            "bands_ms": bands_ms,
            "ring_export_ms_one_sector": ring_ms, "ring_events_one_sector": int(events.size),
            "achieved_GBps": ps * 2 / (kernel_ms / 1000.0) / 1e9,
-           "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous", "pipeline",
+           "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous", "directional", "pipeline",
                                                 "persistent", "both_directions", "two_points")}}
     if cpu:
         log("large tile: CPU oracle walk, 1 thread, every 10th sector")
@@ -186,7 +186,7 @@ def measure_tiles(steps: int, warmup: int, cpu: bool = False) -> dict:
         bands_ms = b.timing().bands_ms
         shape, _ = b.autotune(reps=1)
         log(f"tiles-64 tuned shape: block {shape['block']}, unroll {shape['unroll']}, "
-            f"xcd {shape['xcd_contiguous']}, both directions {shape['both_directions']}, "
+            f"xcd {shape['xcd_contiguous']}, directional {shape['directional']}, both directions {shape['both_directions']}, "
             f"two points {shape['two_points']}")
         for _ in range(warmup):
             b.run(True)
@@ -210,7 +210,7 @@ def measure_tiles(steps: int, warmup: int, cpu: bool = False) -> dict:
             "bands_ms": bands_ms,
             "tiles": len(tiles),
             "sides": sorted({t.width for t in tiles}),
-            "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous",
+            "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous", "directional",
                                                  "pipeline", "persistent", "both_directions",
                                                  "two_points")}}
     if cpu:
@@ -471,7 +471,7 @@ def main() -> int:
     if not args.no_tune and not args.shape:
         shape, table = e.autotune(0, 180, reps=1)
         log(f"autotuned walk shape: block {shape['block']}, unroll {shape['unroll']}, "
-            f"xcd-contiguous {shape['xcd_contiguous']}, pipeline {shape['pipeline']}, "
+            f"xcd-contiguous {shape['xcd_contiguous']}, directional {shape['directional']}, pipeline {shape['pipeline']}, "
             f"persistent {shape['persistent']}, both directions {shape['both_directions']}, "
             f"two points {shape['two_points']} "
             f"({shape['ms']:.2f} ms)")
@@ -581,7 +581,7 @@ def main() -> int:
             "nominal_point_steps_per_step": point_steps,
             "steps_per_band": g.band_deltas_size,
             "real_steps_per_band": g.real_steps,
-            "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous",
+            "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous", "directional",
                                                  "pipeline", "persistent", "both_directions",
                                                  "two_points")},
         },

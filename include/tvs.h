@@ -272,7 +272,11 @@ const char *tvs_status_string(tvs_status status);
 /* ---- launch shape and autotuning of the batched walk --------------------- */
 
 /* Points per work item (threads per block), steps unrolled per chunk,
- * whether consecutive work items stay on one XCD, and whether the next
+ * whether consecutive work items stay on one XCD (`xcd_contiguous`: 0
+ * round-robin over the XCDs, 1 contiguous runs of items per XCD, 2 every
+ * sector's blocks sorted across its band direction and split in 8
+ * contiguous strips, one per XCD; batches of several tiles keep the sorted
+ * order with contiguous runs; other values are refused), and whether the next
  * chunk's heights are requested before the current chunk's arithmetic,
  * whether a grid of a few blocks per CU strides over all work items
  * (`persistent`, which ignores `xcd_contiguous`), and whether one lane walks
@@ -302,7 +306,8 @@ tvs_status tvs_set_walk_shape(tvs_ctx *ctx, const tvs_walk_shape *shape);
 tvs_status tvs_get_walk_shape(const tvs_ctx *ctx, tvs_walk_shape *out);
 
 /* Number of shapes the tuner times: each compiled shape in both item
- * mappings, persistent shapes once. */
+ * mappings (two-points-per-lane shapes also in the directional one),
+ * persistent shapes once. */
 size_t tvs_walk_shape_count(void);
 
 /* Time every compiled shape on the context's own DEM over sectors

@@ -129,10 +129,10 @@ def kernel_info(asm: str, mangled_part: str) -> dict:
 def test_fast_divide_kernels_keep_seven_waves(asm, shape):
     """The fast-divide kernels walk a tile without the fast divides one IEEE
     step at a time, so their registers are those of the fast loops: at most
-    64 VGPRs, seven or more waves per SIMD (the unrolled IEEE loop alone
+    72 VGPRs, seven or more waves per SIMD (the unrolled IEEE loop alone
     needs ~94 VGPRs, five waves)."""
     info = kernel_info(asm, "IsLb0E" + shape + "Lb0E")
-    assert info["TotalNumVgprs"] <= 64 and info["Occupancy"] >= 7, info
+    assert info["TotalNumVgprs"] <= 72 and info["Occupancy"] >= 7, info
     assert info["ScratchSize"] == 0
 
 

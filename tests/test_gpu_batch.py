@@ -105,7 +105,9 @@ def test_batch_shapes_agree(tiles_and_truth):
                                                     (64, 8, False, False, True, False),
                                                     (128, 4, True, False, True, False),
                                                     (128, 8, False, False, False, True),
-                                                    (64, 4, True, False, True, True)]:
+                                                    (64, 4, True, False, True, True),
+                                                    (64, 8, 2, False, True, True),
+                                                    (256, 8, 2, False, False, False)]:
             print(f"batch shape {block}x{unroll} xcd={xcd} pipe={pipe} pair={pair} two={two}",
                   flush=True)
             b.set_walk_shape(block, unroll, xcd, pipe, both_directions=pair, two_points=two)

@@ -43,7 +43,7 @@ def test_large_band_data_bit_exact(large, oracle, sector):
     np.testing.assert_array_equal(got_x.view(np.uint32), want_x.view(np.uint32))
 
 
-SAMPLE = [7, 100]
+SAMPLE = [7, 100, 179]   # 179 opens with a step at distance +0
 
 
 @pytest.fixture(scope="module")
@@ -59,7 +59,8 @@ def sample_truth(large, oracle):
 @pytest.mark.parametrize("shape", [(64, 4, True, True), (128, 8, False, True),
                                    (256, 8, False, False)])
 def test_large_sample_sectors_bit_exact(large, sample_truth, shape):
-    """Sectors 7 and 100 walked alone (one launch each) against the oracle."""
+    """Sectors 7, 100 and 179 walked alone (one launch each) against the
+    oracle."""
     wl, dem, e = large
     block, unroll, pair, two = shape
     sectors = SAMPLE

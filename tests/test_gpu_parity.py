@@ -13,6 +13,7 @@ import pytest
 
 import tvs_amd
 from tvs_amd import demgen, _lib
+from tvs_amd import rings as rings_mod
 
 pytestmark = pytest.mark.gpu
 
@@ -166,10 +167,10 @@ def test_two_points_per_lane_even_and_odd_widths(oracle, w, los):
     with tvs_amd.Engine(w, 100.0, los, 5.0, 1.65, 0) as e:
         e.upload(elev)
         e.compute_bands(0, 180)
-        for block, unroll, pers, pair in ((64, 4, False, True), (128, 8, False, False),
-                                          (64, 8, True, False)):
-            note(f"  shape {block}x{unroll} persistent={pers} pair={pair} two points")
-            e.set_walk_shape(block, unroll, True, False, pers, pair, True)
+        for block, unroll, pers, pair, xcd in ((64, 4, False, True, 1), (128, 8, False, False, 1),
+                                               (64, 8, True, False, 1), (64, 8, False, True, 2)):
+            note(f"  shape {block}x{unroll} persistent={pers} pair={pair} xcd={xcd} two points")
+            e.set_walk_shape(block, unroll, xcd, False, pers, pair, True)
             e.reset_surfaces()
             e.run(0, 180, "batched")
             np.testing.assert_array_equal(e.surfaces().view(np.uint32), cum.view(np.uint32),
@@ -273,6 +274,39 @@ def test_small_tvs_from_host_band_data(small):
             e.set_band(s, deltas[s], dists[s])
         e.run(0, 180, "batched")
         np.testing.assert_array_equal(e.surfaces(), cum)
+
+
+@pytest.mark.parametrize("zeros", [1, 2, 9])
+def test_leading_zero_distances_bit_exact(small, oracle, zeros):
+    """Bands whose first steps sit at distance +0 (3601 x 3601 at 30 m has
+    four such sectors): those steps take the IEEE divide one at a time and
+    the rest of the band the fast loops, in every launch shape, with the
+    oracle's surfaces and ring events."""
+    wl, dem, deltas, dists, _ = small
+    elev = wl.elevations.astype(np.float32)
+    sectors = [3, 47, 95, 150]
+    d = deltas[sectors].copy()
+    x = dists[sectors].copy()
+    x[:, :zeros] = 0.0
+    want, rings = oracle.walk_sectors(dem, elev, wl.observer_height, d, x,
+                                      rings_per_band=64, rings=True, n_threads=THREADS)
+    with make_engine(wl) as e:
+        e.upload(wl.elevations)
+        for i, s in enumerate(sectors):
+            e.set_band(s, d[i], x[i])
+        for block, unroll, pipe, pers, pair, two in SHAPES:
+            note(f"  zeros {zeros} shape {block}x{unroll} pipe={pipe} pers={pers} pair={pair} "
+                 f"two={two}")
+            e.set_walk_shape(block, unroll, True, pipe, pers, pair, two)
+            e.reset_surfaces()
+            for s in sectors:
+                e.run(s, 1, "batched")
+            np.testing.assert_array_equal(e.surfaces().view(np.uint32), want.view(np.uint32),
+                                          err_msg=f"shape {block}x{unroll} pair={pair} two={two}")
+        s0 = sectors[1]
+        counts, events = e.compact_rings(s0)
+        np.testing.assert_array_equal(rings_mod.to_records(e.geometry, counts, events, 64),
+                                      rings[1])
 
 
 def test_small_gpu_run_mirror(small, oracle):
@@ -498,10 +532,12 @@ SHAPES = [  # (block, unroll, pipeline, persistent, both directions, two points 
     (64, 8, False, True, False, True)]
 
 
-@pytest.mark.parametrize("xcd", [True, False])
+# xcd: 0 round-robin items, 1 XCD-contiguous, 2 XCD-contiguous in the
+# directional order (each sector's blocks sorted across its band direction).
+@pytest.mark.parametrize("xcd", [1, 0, 2])
 def test_every_walk_shape_bit_exact(small, xcd):
-    """Each compiled launch shape of the batched walk gives the oracle's
-    surfaces (the tuner may pick any of them)."""
+    """Each compiled launch shape of the batched walk, in each item order,
+    gives the oracle's surfaces (the tuner may pick any of them)."""
     wl, dem, _, _, cum = small
     with make_engine(wl) as e:
         e.upload(wl.elevations)
@@ -537,7 +573,7 @@ def small_fractional(request, oracle):
     return wl, elev, cum, request.param == "spaced"
 
 
-@pytest.mark.parametrize("xcd", [True, False])
+@pytest.mark.parametrize("xcd", [1, 0, 2])
 def test_every_walk_shape_bit_exact_f32(small_fractional, xcd):
     """The f32-height kernels of every compiled shape, on non-integer
     heights (four-operation divide, or IEEE with a tiny height), give the
@@ -556,3 +592,18 @@ def test_every_walk_shape_bit_exact_f32(small_fractional, xcd):
             np.testing.assert_array_equal(e.surfaces(), cum,
                                           err_msg=f"shape {block}x{unroll} pipe={pipe} "
                                                   f"pers={pers} pair={pair} two={two}")
+
+
+def test_directional_order_shape_round_trip(small):
+    """xcd_contiguous 2 selects the directional item order and reads back as
+    such; other values above 1 are refused."""
+    wl = small[0]
+    with make_engine(wl) as e:
+        e.set_walk_shape(64, 8, 2, False, False, True, True)
+        shape = e.walk_shape()
+        assert shape["xcd_contiguous"] and shape["directional"]
+        e.set_walk_shape(64, 8, 1, False, False, True, True)
+        assert not e.walk_shape()["directional"]
+        with pytest.raises(tvs_amd.TvsError) as err:
+            e.set_walk_shape(64, 8, 3, False, False, True, True)
+        assert err.value.status == _lib.ERR_INVALID_ARGUMENT

@@ -347,6 +347,23 @@ tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t cap
     d.ring_counts = nullptr;
     d.ring_offsets = nullptr;
     d.ring_events = nullptr;
+    d.order = nullptr;
+    if (cfg.directional && cfg.xcd_contiguous && !cfg.persistent) {
+        const uint64_t key = (1ull << 63) | (uint64_t(cfg.twin) << 48) |
+                             (uint64_t(cfg.block) << 32) | (uint64_t(d.w) << 16) | d.T;
+        if (c->order_key != key) {
+            const std::vector<uint32_t> order =
+                directional_walk_order(d.T, d.w, cfg, kBandSlots);
+            dev_free(c->d_order);
+            c->order_key = 0;
+            TVS_HIP(c, dev_alloc(&c->d_order, order.size()), "walk order");
+            TVS_HIP(c, hipMemcpy(c->d_order, order.data(), order.size() * sizeof(uint32_t),
+                                 hipMemcpyHostToDevice),
+                    "walk order");
+            c->order_key = key;
+        }
+        d.order = c->d_order;
+    }
     *out = d;
     return TVS_OK;
 }
@@ -478,6 +495,7 @@ tvs_status tvs_destroy(tvs_ctx *c) {
     dev_free(c->d_ring_counts);
     dev_free(c->d_ring_offsets);
     dev_free(c->d_ring_events);
+    dev_free(c->d_order);
     dev_free(c->d_axes);
     dev_free(c->d_select_status);
     dev_free(c->d_begin);
@@ -801,7 +819,8 @@ tvs_status tvs_run_sectors(tvs_ctx *c, uint32_t first, uint32_t count, tvs_metho
         TVS_HIP(c, hipEventRecord(c->ev[0], c->stream), "event");
         TVS_HIP(c, hipEventRecord(hist[0], c->stream), "event");
         TVS_HIP(c, tvs::launch_walk_tiles(c->stream, c->d_tile, c->d_begin, 1, n_items,
-                                          c->has_i16, capture > 0, d.fast == 0u, cfg),
+                                          c->has_i16, capture > 0, d.fast == 0u, cfg,
+                                          d.blocks_per_pass),
                 "batched walk");
         TVS_HIP(c, hipEventRecord(c->ev[1], c->stream), "event");
         TVS_HIP(c, hipEventRecord(hist[1], c->stream), "event");
@@ -879,7 +898,8 @@ tvs_status tvs_compact_rings(tvs_ctx *c, uint32_t sector, uint32_t *counts, size
         c->ring_cache_gen = ~uint64_t(0);
         TVS_HIP(c, hipMemcpyAsync(c->d_tile, &d, sizeof d, hipMemcpyHostToDevice, c->stream), "descriptor");
         TVS_HIP(c, tvs::launch_walk_tiles(c->stream, c->d_tile, c->d_begin, 1, n_items,
-                                          c->has_i16, true, d.fast == 0u, cfg),
+                                          c->has_i16, true, d.fast == 0u, cfg,
+                                          d.blocks_per_pass),
                 "ring count walk");
         TVS_HIP(c, tvs::launch_ring_offsets(c->stream, c->d_ring_counts,
                                             static_cast<uint32_t>(bands), c->d_ring_offsets),
@@ -913,7 +933,8 @@ tvs_status tvs_compact_rings(tvs_ctx *c, uint32_t sector, uint32_t *counts, size
     d.ring_events = c->d_ring_events;
     TVS_HIP(c, hipMemcpyAsync(c->d_tile, &d, sizeof d, hipMemcpyHostToDevice, c->stream), "descriptor");
     TVS_HIP(c, tvs::launch_walk_tiles(c->stream, c->d_tile, c->d_begin, 1, n_items, c->has_i16,
-                                      true, d.fast == 0u, cfg), "ring write walk");
+                                      true, d.fast == 0u, cfg, d.blocks_per_pass),
+            "ring write walk");
     if (total)
         TVS_HIP(c, hipMemcpyAsync(events, c->d_ring_events, total * sizeof(uint32_t),
                                   hipMemcpyDeviceToHost, c->stream), "ring events");

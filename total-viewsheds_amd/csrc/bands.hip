@@ -231,6 +231,7 @@ __global__ __launch_bounds__(kScanThreads) void step_table_kernel(
     __shared__ int32_t part[kScanThreads];
     __shared__ int32_t red_min[kScanThreads / 64], red_max[kScanThreads / 64];
     __shared__ uint32_t red_fast[kScanThreads / 64], red_tail[kScanThreads / 64];
+    __shared__ uint32_t red_reg[kScanThreads / 64], red_fast2[kScanThreads / 64];
     const uint32_t slot = first_slot + blockIdx.x;
     const size_t tb = static_cast<size_t>(slot) * tab.stride;
     const int32_t *delta = tab.delta + tb;
@@ -252,6 +253,7 @@ __global__ __launch_bounds__(kScanThreads) void step_table_kernel(
     int32_t run = tid ? part[tid - 1] : 0;
     int32_t mn = 0x7fffffff, mx = -0x7fffffff - 1;
     uint32_t first_irregular = n_steps;
+    uint32_t first_regular = n_steps;
     uint32_t last_nonzero = 0;  // one past the last step outside the zero tail
     for (uint32_t i = begin; i < end; ++i) {
         run += delta[i];
@@ -265,6 +267,7 @@ __global__ __launch_bounds__(kScanThreads) void step_table_kernel(
         mn = run < mn ? run : mn;
         mx = run > mx ? run : mx;
         if (!regular_distance(d) && i < first_irregular) first_irregular = i;
+        if (regular_distance(d) && i < first_regular) first_regular = i;
         if (delta[i] != 0 || __float_as_uint(d) != 0u) last_nonzero = i + 1;
     }
     // Wave then block reductions.
@@ -272,23 +275,47 @@ __global__ __launch_bounds__(kScanThreads) void step_table_kernel(
         mn = min(mn, __shfl_xor(mn, o));
         mx = max(mx, __shfl_xor(mx, o));
         first_irregular = min(first_irregular, static_cast<uint32_t>(__shfl_xor(static_cast<int>(first_irregular), o)));
+        first_regular = min(first_regular, static_cast<uint32_t>(__shfl_xor(static_cast<int>(first_regular), o)));
         last_nonzero = max(last_nonzero, static_cast<uint32_t>(__shfl_xor(static_cast<int>(last_nonzero), o)));
     }
     if ((tid & 63u) == 0) {
         red_min[tid >> 6] = mn;
         red_max[tid >> 6] = mx;
         red_fast[tid >> 6] = first_irregular;
+        red_reg[tid >> 6] = first_regular;
         red_tail[tid >> 6] = last_nonzero;
     }
     __syncthreads();
     // Block-wide results (every thread needs the bias).
     int32_t lo = 0, hi = 0;   // the PoV itself is on every band
-    uint32_t fast = n_steps, tail = 0;
+    uint32_t fast = n_steps, tail = 0, reg = n_steps;
     for (uint32_t wv = 0; wv < kScanThreads / 64; ++wv) {
         lo = min(lo, red_min[wv]);
         hi = max(hi, red_max[wv]);
         fast = min(fast, red_fast[wv]);
+        reg = min(reg, red_reg[wv]);
         tail = max(tail, red_tail[wv]);
+    }
+    // Leading irregular steps (a first band point at distance +0, say) are
+    // walked one at a time before the fast range: head = the first regular
+    // step when the band opens with irregular ones, and the fast range ends
+    // at the first irregular step after it.
+    const uint32_t head = fast < reg ? reg : 0u;
+    if (head != 0u) {
+        uint32_t irr = n_steps;
+        for (uint32_t i = begin > head ? begin : head; i < end; ++i)
+            if (!regular_distance(dist[i])) {
+                irr = i;
+                break;
+            }
+        for (uint32_t o = 32; o > 0; o >>= 1)
+            irr = min(irr, static_cast<uint32_t>(__shfl_xor(static_cast<int>(irr), o)));
+        if ((tid & 63u) == 0) red_fast2[tid >> 6] = irr;
+    }
+    __syncthreads();
+    if (head != 0u) {
+        fast = n_steps;
+        for (uint32_t wv = 0; wv < kScanThreads / 64; ++wv) fast = min(fast, red_fast2[wv]);
     }
     const int32_t bias = max(hi, -lo);
     // Byte offsets of int16 elements: 2 * (bias +/- off), plus 2 * copy_offset
@@ -306,6 +333,7 @@ __global__ __launch_bounds__(kScanThreads) void step_table_kernel(
         m.min_off = lo;
         m.max_off = hi;
         m.n_fast = fast > tail ? tail : fast;
+        m.head = head > m.n_fast ? m.n_fast : head;
         m.tail_start = tail;
         m.status = 0;
         m.bias = static_cast<uint32_t>(bias);

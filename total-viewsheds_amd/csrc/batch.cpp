@@ -266,9 +266,12 @@ tvs_status tvs_batch_compute_bands(tvs_batch *b) {
 
 tvs_status tvs_batch_set_walk_shape(tvs_batch *b, const tvs_walk_shape *shape) {
     if (!b || !shape) return bfail(b, TVS_ERR_INVALID_ARGUMENT, "null argument");
+    if (shape->xcd_contiguous > 2u)
+        return bfail(b, TVS_ERR_INVALID_ARGUMENT, "xcd_contiguous must be 0, 1 or 2");
     tvs::WalkConfig cfg{shape->block, shape->unroll, shape->xcd_contiguous != 0,
                         shape->pipeline != 0, shape->persistent != 0,
-                        shape->both_directions != 0, shape->two_points != 0};
+                        shape->both_directions != 0, shape->two_points != 0,
+                        shape->xcd_contiguous == 2u};
     if (!tvs::walk_shape_supported(cfg))
         return bfail(b, TVS_ERR_INVALID_ARGUMENT, "walk shape not compiled");
     b->cfg = cfg;
@@ -314,7 +317,7 @@ tvs_status tvs_batch_autotune(tvs_batch *b, uint32_t reps, tvs_walk_shape *resul
         }
         ms /= static_cast<float>(reps);
         if (results && k < max_results)
-            results[k] = tvs_walk_shape{cfg.block, cfg.unroll, cfg.xcd_contiguous ? 1u : 0u,
+            results[k] = tvs_walk_shape{cfg.block, cfg.unroll, tvs::xcd_mode(cfg),
                                         cfg.pipeline ? 1u : 0u, cfg.persistent ? 1u : 0u,
                                         cfg.pair ? 1u : 0u, cfg.twin ? 1u : 0u, ms};
         ++k;
@@ -330,7 +333,7 @@ tvs_status tvs_batch_autotune(tvs_batch *b, uint32_t reps, tvs_walk_shape *resul
     if (st != TVS_OK) return st;
     if (n_results) *n_results = k;
     if (best)
-        *best = tvs_walk_shape{winner.block, winner.unroll, winner.xcd_contiguous ? 1u : 0u,
+        *best = tvs_walk_shape{winner.block, winner.unroll, tvs::xcd_mode(winner),
                                winner.pipeline ? 1u : 0u, winner.persistent ? 1u : 0u,
                                winner.pair ? 1u : 0u, winner.twin ? 1u : 0u, winner_ms};
     return TVS_OK;

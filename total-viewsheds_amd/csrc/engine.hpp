@@ -43,9 +43,12 @@ inline uint64_t dem_span(uint64_t n) { return dem_copy_offset(n) + n; }
 // which makes the parity 0; on even-width DEMs it is that of L.
 inline uint32_t pair_parity(uint32_t w, uint32_t L) { return (w & 1u) ? 0u : (L & 1u); }
 
-// Steps of one sector's walk are split in three ranges, each a uniform
+// Steps of one sector's walk are split in four ranges, each a uniform
 // (whole-grid) property of the sector:
-//   [0, n_fast)            regular steps: d normal, the fast divide holds
+//   [0, head)              leading irregular steps (d outside the regular
+//                          range, e.g. a first band point at distance +0):
+//                          IEEE divide; head is 0 when step 0 is regular
+//   [head, n_fast)         regular steps: d normal, the fast divide holds
 //   [n_fast, tail_start)   anything else: per-step choice of divide
 //   [tail_start, n_steps)  the zero tail of band_of_sight.rs:67-68 (delta 0,
 //                          distance +0): the point and the angle repeat.
@@ -56,7 +59,8 @@ struct SectorMeta {
     uint32_t tail_start;
     uint32_t status;     // non-zero: band compile hit a reference panic site
     uint32_t bias;       // max |cumulative offset|: positions below are bias +/- off
-    uint32_t pad[2];
+    uint32_t head;
+    uint32_t pad;
 };
 
 // Device-resident per-sector step tables (structure of arrays, `stride`
@@ -110,6 +114,8 @@ struct TileDesc {
     uint32_t *ring_counts;       // u32[2N]: ring_id per band (kRingCount)
     const uint64_t *ring_offsets;  // u64[2N]: first event of each band (kRingCompact)
     uint32_t *ring_events;       // packed event ids (kRingCompact)
+    const uint32_t *order;       // per sector slot, blocks_per_pass block indices in walk
+                                 // order (null: ascending)
 };
 
 // Ring capture modes of the tile walk.
@@ -127,7 +133,18 @@ struct WalkConfig {
     bool persistent = false;  // a few blocks per CU stride over all items
     bool pair = false;        // one lane walks both bands (forward, backward) of a point
     bool twin = false;        // one lane walks two adjacent points of a row
+    // With xcd_contiguous: the work items of each sector taken in the order
+    // of their position across the sector's band direction, so that the
+    // items in flight on one XCD form a strip along that direction and
+    // their bands share the DEM rows and columns its L2 holds.
+    bool directional = false;
 };
+
+// tvs_walk_shape::xcd_contiguous of a shape: 0 round-robin items, 1
+// XCD-contiguous, 2 XCD-contiguous in directional order.
+inline uint32_t xcd_mode(const WalkConfig &cfg) {
+    return cfg.directional ? 2u : cfg.xcd_contiguous ? 1u : 0u;
+}
 
 // The shape used until a caller sets one or runs the tuner: the fastest
 // measured on the 1201 x 1201 workload for each height type (int16: 64
@@ -225,6 +242,8 @@ struct tvs_ctx {
     uint32_t *d_ring_counts = nullptr;    // compact ring export
     uint64_t *d_ring_offsets = nullptr;   // u64[2N + 1 + tiles]: offsets, total, scan scratch
     uint32_t *d_ring_events = nullptr;
+    uint32_t *d_order = nullptr;          // directional item order (see WalkConfig)
+    uint64_t order_key = 0;               // (T, w, block, twin) it was built for; 0: none
     size_t ring_events_cap = 0;
     // Changes with every upload and band change; the compact ring export
     // keeps the counts and offsets of the last sector it counted under it.
@@ -273,16 +292,22 @@ WalkConfig effective_walk_config(const tvs_ctx *c);
 hipError_t launch_short_div_check(hipStream_t s, const StepTables &tab, uint32_t first,
                              uint32_t count, const float *domain, uint32_t n_domain,
                              uint32_t *bad);
-// Every compiled shape the tuners time: both item mappings, persistent
-// shapes once.
+// Every compiled shape the tuners time: both item mappings (and the
+// directional order for two points per lane), persistent shapes once.
 std::vector<WalkConfig> compiled_walk_shapes();
+// Per sector slot in [0, n_slots): the walk order of the launch shape's
+// blocks_per_pass blocks for WalkConfig::directional.
+std::vector<uint32_t> directional_walk_order(uint32_t T, uint32_t w, const WalkConfig &cfg,
+                                             uint32_t n_slots);
 // ieee_only: no tile of the launch has the fast divides (TileDesc::fast
 // == 0); the kernel then unrolls the IEEE-divide loop.  Either value gives
-// the same results.
+// the same results.  pass_items: the one tile's blocks_per_pass, which lets
+// a directional launch split every pass across the XCDs (0: not known,
+// e.g. several tiles; the directional order then runs XCD-contiguous).
 hipError_t launch_walk_tiles(hipStream_t s, const TileDesc *tiles,
                              const uint32_t *item_begin, uint32_t n_tiles,
                              uint32_t n_items, bool i16, bool capture, bool ieee_only,
-                             const WalkConfig &cfg);
+                             const WalkConfig &cfg, uint32_t pass_items = 0);
 hipError_t launch_reduce_tiles(hipStream_t s, const TileDesc *tiles,
                                const uint32_t *point_begin, uint32_t n_tiles,
                                uint32_t n_points);

@@ -7,6 +7,8 @@
 // becomes the context's shape.  The cumulative surfaces are saved and
 // restored around the timing runs.
 #include <algorithm>
+#include <cmath>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -17,12 +19,73 @@ namespace tvs {
 std::vector<WalkConfig> compiled_walk_shapes() {
     std::vector<WalkConfig> out;
     // A persistent grid strides over the items itself: one item mapping.
-#define TVS_SHAPE(B, U, P, S, D, W)                                   \
-    if (!S) out.push_back(WalkConfig{B, U, true, P, S, D, W});   \
+    // The directional order is timed for the two-points-per-lane shapes
+    // (the int16 walk's fastest family).
+#define TVS_SHAPE(B, U, P, S, D, W)                                        \
+    if (!S) out.push_back(WalkConfig{B, U, true, P, S, D, W});        \
+    if (!S && W) out.push_back(WalkConfig{B, U, true, P, S, D, W, true}); \
     out.push_back(WalkConfig{B, U, false, P, S, D, W});
     TVS_WALK_SHAPES(TVS_SHAPE)
 #undef TVS_SHAPE
     return out;
+}
+
+// Block order of a directional launch, per sector slot: the blocks of the
+// launch shape (as the walk maps them to points) sorted by the projection of
+// their centre on the normal of the sector's band direction.  The band of
+// sector s runs along (column, row) = (sin a, cos a), a = s + 0.001 degrees
+// (axes.rs:89-100 and the band data it yields: sector 0 steps through rows,
+// sector 90 along a row).  Slots past the 180 sectors keep ascending order.
+std::vector<uint32_t> directional_walk_order(uint32_t T, uint32_t w, const WalkConfig &cfg,
+                                             uint32_t n_slots) {
+    const uint32_t bpp = walk_blocks_per_pass(T, cfg);
+    const uint32_t B = cfg.block;
+    std::vector<double> row(bpp), col(bpp);
+    auto at = [&](uint32_t k, double r0, double c0, double r1, double c1) {
+        row[k] = 0.5 * (r0 + r1);
+        col[k] = 0.5 * (c0 + c1);
+    };
+    if (cfg.twin) {
+        const uint64_t half = T / 2u, n_pairs = uint64_t(T) * half;
+        const uint32_t twin_blocks = static_cast<uint32_t>((n_pairs + B - 1) / B);
+        auto pair_rc = [&](uint64_t p, double &r, double &cc) {
+            const uint64_t rr = p / half;
+            r = static_cast<double>(rr);
+            cc = 2.0 * static_cast<double>(p - rr * half) + ((w & rr & 1u) ? 1.5 : 0.5);
+        };
+        for (uint32_t k = 0; k < bpp; ++k) {
+            double r0, c0, r1, c1;
+            if (k < twin_blocks) {
+                const uint64_t p0 = uint64_t(k) * B;
+                pair_rc(p0, r0, c0);
+                pair_rc(std::min<uint64_t>(p0 + B, n_pairs) - 1u, r1, c1);
+            } else {   // each row's left-over column (odd T), one point per lane
+                const uint64_t i0 = uint64_t(k - twin_blocks) * B;
+                r0 = static_cast<double>(i0);
+                r1 = static_cast<double>(std::min<uint64_t>(i0 + B, T) - 1u);
+                c0 = c1 = 0.5 * (T - 1);
+            }
+            at(k, r0, c0, r1, c1);
+        }
+    } else {
+        const uint64_t N = uint64_t(T) * T;
+        for (uint32_t k = 0; k < bpp; ++k) {
+            const uint64_t t0 = uint64_t(k) * B, t1 = std::min<uint64_t>(t0 + B, N) - 1u;
+            at(k, double(t0 / T), double(t0 % T), double(t1 / T), double(t1 % T));
+        }
+    }
+    std::vector<uint32_t> order(static_cast<size_t>(n_slots) * bpp);
+    std::vector<double> key(bpp);
+    for (uint32_t s = 0; s < n_slots; ++s) {
+        uint32_t *o = order.data() + static_cast<size_t>(s) * bpp;
+        std::iota(o, o + bpp, 0u);
+        if (s >= kSectorSteps) continue;
+        const double a = (s + static_cast<double>(kSectorShift)) * (M_PI / 180.0);
+        const double ns = std::sin(a), nc = std::cos(a);
+        for (uint32_t k = 0; k < bpp; ++k) key[k] = col[k] * nc - row[k] * ns;
+        std::stable_sort(o, o + bpp, [&](uint32_t x, uint32_t y) { return key[x] < key[y]; });
+    }
+    return order;
 }
 
 }  // namespace tvs
@@ -40,9 +103,12 @@ extern "C" {
 
 tvs_status tvs_set_walk_shape(tvs_ctx *c, const tvs_walk_shape *shape) {
     if (!c || !shape) return TVS_ERR_INVALID_ARGUMENT;
+    if (shape->xcd_contiguous > 2u)
+        return tfail(c, TVS_ERR_INVALID_ARGUMENT, "xcd_contiguous must be 0, 1 or 2");
     tvs::WalkConfig cfg{shape->block, shape->unroll, shape->xcd_contiguous != 0,
                         shape->pipeline != 0, shape->persistent != 0,
-                        shape->both_directions != 0, shape->two_points != 0};
+                        shape->both_directions != 0, shape->two_points != 0,
+                        shape->xcd_contiguous == 2u};
     if (!tvs::walk_shape_supported(cfg))
         return tfail(c, TVS_ERR_INVALID_ARGUMENT, "walk shape not compiled");
     c->walk_cfg = cfg;
@@ -55,7 +121,7 @@ tvs_status tvs_get_walk_shape(const tvs_ctx *c, tvs_walk_shape *out) {
     const tvs::WalkConfig cfg = tvs::effective_walk_config(c);
     out->block = cfg.block;
     out->unroll = cfg.unroll;
-    out->xcd_contiguous = cfg.xcd_contiguous ? 1u : 0u;
+    out->xcd_contiguous = tvs::xcd_mode(cfg);
     out->pipeline = cfg.pipeline ? 1u : 0u;
     out->persistent = cfg.persistent ? 1u : 0u;
     out->both_directions = cfg.pair ? 1u : 0u;
@@ -107,7 +173,7 @@ tvs_status tvs_autotune(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t rep
         if (st != TVS_OK) break;
         const float ms = total / static_cast<float>(reps);
         if (results && k < max_results)
-            results[k] = tvs_walk_shape{cfg.block, cfg.unroll, cfg.xcd_contiguous ? 1u : 0u,
+            results[k] = tvs_walk_shape{cfg.block, cfg.unroll, tvs::xcd_mode(cfg),
                                         cfg.pipeline ? 1u : 0u, cfg.persistent ? 1u : 0u,
                                         cfg.pair ? 1u : 0u, cfg.twin ? 1u : 0u, ms};
         ++k;
@@ -128,7 +194,7 @@ tvs_status tvs_autotune(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t rep
     if (e != hipSuccess) return tfail(c, TVS_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
     if (n_results) *n_results = k;
     if (best)
-        *best = tvs_walk_shape{winner.block, winner.unroll, winner.xcd_contiguous ? 1u : 0u,
+        *best = tvs_walk_shape{winner.block, winner.unroll, tvs::xcd_mode(winner),
                                winner.pipeline ? 1u : 0u, winner.persistent ? 1u : 0u,
                                winner.pair ? 1u : 0u, winner.twin ? 1u : 0u, winner_ms};
     return TVS_OK;

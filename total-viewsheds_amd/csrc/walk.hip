@@ -232,6 +232,7 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
     const uint32_t pov = (t / d.T + d.L) * d.w + (t % d.T + d.L);
     const uint32_t pov_bytes = pov * static_cast<uint32_t>(sizeof(H));
     const float h0 = static_cast<float>(global_ptr(elev)[pov]) + d.obs;
+    const uint32_t head = const_ptr(d.meta)[slot].head;
     const uint32_t n_fast = const_ptr(d.meta)[slot].n_fast;
     const uint32_t tail_start = const_ptr(d.meta)[slot].tail_start;
     const uint32_t bias = const_ptr(d.meta)[slot].bias;
@@ -290,64 +291,76 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
         }
     };
 
-    uint32_t k = 0;
-    const uint32_t fast_end = n_fast - n_fast % UNROLL;
-    auto chunk = [&](const float (&h)[UNROLL], uint32_t k0) {
-#pragma unroll
-        for (int u = 0; u < UNROLL; ++u) {
-            const float dh = h[u] - h0;
-            const float q = step_div<DIV>(dh, dist[k0 + u], rcp[k0 + u], rcp_lo[k0 + u]);
-            const float angle = q - curv[k0 + u];
-            if (!CAPTURE) {
-                step_update(angle, wt[k0 + u], lanes, max_angle, surface);
-                continue;
-            }
-            const bool v = angle > max_angle;
-            if (capture && v != visible) event(pos[k0 + u]);
-            surface = add_if(v, surface, wt[k0 + u]);
-            max_angle = v ? angle : max_angle;
-            visible = v;
-        }
-    };
-    if (PIPE) {
-        // Heights of the next chunk are requested before this chunk's
-        // arithmetic; the last iteration re-reads its own chunk (in range).
-        if (fast_end) {
-            float h[UNROLL];
-#pragma unroll
-            for (int u = 0; u < UNROLL; ++u) h[u] = load_height<H>(rsrc, pov_bytes, soff<H>(pos[u]));
-            for (; k < fast_end; k += UNROLL) {
-                const uint32_t kn = k + UNROLL < fast_end ? k + UNROLL : k;
-                float hn[UNROLL];
-#pragma unroll
-                for (int u = 0; u < UNROLL; ++u)
-                    hn[u] = load_height<H>(rsrc, pov_bytes, soff<H>(pos[kn + u]));
-                chunk(h, k);
-#pragma unroll
-                for (int u = 0; u < UNROLL; ++u) h[u] = hn[u];
-            }
-        }
-    } else {
-        for (; k < fast_end; k += UNROLL) {
-            float h[UNROLL];
-#pragma unroll
-            for (int u = 0; u < UNROLL; ++u) h[u] = load_height<H>(rsrc, pov_bytes, soff<H>(pos[k + u]));
-            chunk(h, k);
-        }
-    }
-    for (; k < tail_start; ++k) {
+    // One step with the IEEE divide (fast_k false) or the sector's divide.
+    auto single = [&](uint32_t k, bool fast_k) {
         const uint32_t o = pos[k];
         const float dk = dist[k];
         const float dh = load_height<H>(rsrc, pov_bytes, soff<H>(o)) - h0;
-        const float q = (DIV != 0 && k < n_fast) ? step_div<DIV>(dh, dk, rcp[k], rcp_lo[k])
-                                                 : dh / dk;
+        const float q = (DIV != 0 && fast_k) ? step_div<DIV>(dh, dk, rcp[k], rcp_lo[k]) : dh / dk;
         const float angle = q - curv[k];
         const bool v = angle > max_angle;
         if (CAPTURE && capture && v != visible) event(o);
         surface = add_if(v, surface, wt[k]);
         max_angle = v ? angle : max_angle;
         visible = v;
+    };
+    uint32_t k = 0;
+    for (; k < head; ++k) single(k, false);
+    // The fast range [head, n_fast) in whole chunks, through the tables
+    // from step head on (f*), indexed from 0.
+    const uint32_t fast_end = (n_fast - head) - (n_fast - head) % UNROLL;
+    const TVS_CONST uint32_t *__restrict__ fpos = pos + head;
+    const TVS_CONST float *__restrict__ fdist = dist + head;
+    const TVS_CONST float *__restrict__ frcp = rcp + head;
+    const TVS_CONST float *__restrict__ frcp_lo = rcp_lo + head;
+    const TVS_CONST float *__restrict__ fcurv = curv + head;
+    const TVS_CONST float *__restrict__ fwt = wt + head;
+    auto chunk = [&](const float (&h)[UNROLL], uint32_t k0) {
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            const float dh = h[u] - h0;
+            const float q = step_div<DIV>(dh, fdist[k0 + u], frcp[k0 + u], frcp_lo[k0 + u]);
+            const float angle = q - fcurv[k0 + u];
+            if (!CAPTURE) {
+                step_update(angle, fwt[k0 + u], lanes, max_angle, surface);
+                continue;
+            }
+            const bool v = angle > max_angle;
+            if (capture && v != visible) event(fpos[k0 + u]);
+            surface = add_if(v, surface, fwt[k0 + u]);
+            max_angle = v ? angle : max_angle;
+            visible = v;
+        }
+    };
+    uint32_t j = 0;
+    if (PIPE) {
+        // Heights of the next chunk are requested before this chunk's
+        // arithmetic; the last iteration re-reads its own chunk (in range).
+        if (fast_end) {
+            float h[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) h[u] = load_height<H>(rsrc, pov_bytes, soff<H>(fpos[u]));
+            for (; j < fast_end; j += UNROLL) {
+                const uint32_t jn = j + UNROLL < fast_end ? j + UNROLL : j;
+                float hn[UNROLL];
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u)
+                    hn[u] = load_height<H>(rsrc, pov_bytes, soff<H>(fpos[jn + u]));
+                chunk(h, j);
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u) h[u] = hn[u];
+            }
+        }
+    } else {
+        for (; j < fast_end; j += UNROLL) {
+            float h[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) h[u] = load_height<H>(rsrc, pov_bytes, soff<H>(fpos[j + u]));
+            chunk(h, j);
+        }
     }
+    k += fast_end;
+    for (; k < tail_start; ++k) single(k, k < n_fast);
     if (k < d.n_steps) {
         // Zero tail: delta 0 keeps the point, distance +0 gives the same
         // angle (dh/0 - 0: +inf, -inf or NaN) at every step, and the weight
@@ -395,6 +408,7 @@ __device__ __forceinline__ void walk_pair(const TileDesc &d, uint32_t rel, uint3
     const uint32_t pov = (t / d.T + d.L) * d.w + (t % d.T + d.L);
     const uint32_t pov_bytes = pov * static_cast<uint32_t>(sizeof(H));
     const float h0 = static_cast<float>(global_ptr(elev)[pov]) + d.obs;
+    const uint32_t head = const_ptr(d.meta)[slot].head;
     const uint32_t n_fast = const_ptr(d.meta)[slot].n_fast;
     const uint32_t tail_start = const_ptr(d.meta)[slot].tail_start;
     const uint32_t bias = const_ptr(d.meta)[slot].bias;
@@ -414,35 +428,47 @@ __device__ __forceinline__ void walk_pair(const TileDesc &d, uint32_t rel, uint3
 
     float max_f = kMaxAngle, max_b = kMaxAngle;
     float surface_f = 0.0f, surface_b = 0.0f;
+    // One step with the IEEE divide (fast_k false) or the sector's divide.
+    auto single = [&](uint32_t k, bool fast_k) {
+        const float dk = dist[k];
+        const float dhf = load_height<H>(rsrc, pov_bytes, soff<H>(pos_f[k])) - h0;
+        const float dhb = load_height<H>(rsrc, pov_bytes, soff<H>(pos_b[k])) - h0;
+        const float qf = (DIV != 0 && fast_k) ? step_div<DIV>(dhf, dk, rcp[k], rcp_lo[k]) : dhf / dk;
+        const float qb = (DIV != 0 && fast_k) ? step_div<DIV>(dhb, dk, rcp[k], rcp_lo[k]) : dhb / dk;
+        pair_update(qf - curv[k], qb - curv[k], wt[k], lanes, max_f, max_b, surface_f,
+                    surface_b);
+    };
     uint32_t k = 0;
-    const uint32_t fast_end = n_fast - n_fast % UNROLL;
-    for (; k < fast_end; k += UNROLL) {
+    for (; k < head; ++k) single(k, false);
+    // The fast range [head, n_fast) in whole chunks, through the tables
+    // from step head on (f*), indexed from 0.
+    const uint32_t fast_end = (n_fast - head) - (n_fast - head) % UNROLL;
+    const TVS_CONST uint32_t *__restrict__ fpos_f = pos_f + head;
+    const TVS_CONST uint32_t *__restrict__ fpos_b = pos_b + head;
+    const TVS_CONST float *__restrict__ fdist = dist + head;
+    const TVS_CONST float *__restrict__ frcp = rcp + head;
+    const TVS_CONST float *__restrict__ frcp_lo = rcp_lo + head;
+    const TVS_CONST float *__restrict__ fcurv = curv + head;
+    const TVS_CONST float *__restrict__ fwt = wt + head;
+    for (uint32_t j = 0; j < fast_end; j += UNROLL) {
         float hf[UNROLL], hb[UNROLL];
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
-            hf[u] = load_height<H>(rsrc, pov_bytes, soff<H>(pos_f[k + u]));
-            hb[u] = load_height<H>(rsrc, pov_bytes, soff<H>(pos_b[k + u]));
+            hf[u] = load_height<H>(rsrc, pov_bytes, soff<H>(fpos_f[j + u]));
+            hb[u] = load_height<H>(rsrc, pov_bytes, soff<H>(fpos_b[j + u]));
         }
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
             const float dhf = hf[u] - h0;
             const float dhb = hb[u] - h0;
-            const float qf = step_div<DIV>(dhf, dist[k + u], rcp[k + u], rcp_lo[k + u]);
-            const float qb = step_div<DIV>(dhb, dist[k + u], rcp[k + u], rcp_lo[k + u]);
-            pair_update(qf - curv[k + u], qb - curv[k + u], wt[k + u], lanes, max_f, max_b,
+            const float qf = step_div<DIV>(dhf, fdist[j + u], frcp[j + u], frcp_lo[j + u]);
+            const float qb = step_div<DIV>(dhb, fdist[j + u], frcp[j + u], frcp_lo[j + u]);
+            pair_update(qf - fcurv[j + u], qb - fcurv[j + u], fwt[j + u], lanes, max_f, max_b,
                         surface_f, surface_b);
         }
     }
-    for (; k < tail_start; ++k) {
-        const float dk = dist[k];
-        const float dhf = load_height<H>(rsrc, pov_bytes, soff<H>(pos_f[k])) - h0;
-        const float dhb = load_height<H>(rsrc, pov_bytes, soff<H>(pos_b[k])) - h0;
-        const bool fast_k = DIV != 0 && k < n_fast;
-        const float qf = fast_k ? step_div<DIV>(dhf, dk, rcp[k], rcp_lo[k]) : dhf / dk;
-        const float qb = fast_k ? step_div<DIV>(dhb, dk, rcp[k], rcp_lo[k]) : dhb / dk;
-        pair_update(qf - curv[k], qb - curv[k], wt[k], lanes, max_f, max_b, surface_f,
-                    surface_b);
-    }
+    k += fast_end;
+    for (; k < tail_start; ++k) single(k, k < n_fast);
     if (active) {
         TVS_GLOBAL float *out = global_ptr(d.partial) + static_cast<size_t>(2u * rel) * d.N + t;
         out[0] = surface_f;
@@ -493,6 +519,7 @@ __device__ __forceinline__ void walk_twin(const TileDesc &d, uint32_t rel, bool 
     const uint32_t pov_bytes = pov * static_cast<uint32_t>(sizeof(H));
     const float h0a = static_cast<float>(global_ptr(elev)[pov]) + d.obs;
     const float h0b = static_cast<float>(global_ptr(elev)[pov + 1u]) + d.obs;
+    const uint32_t head = const_ptr(d.meta)[slot].head;
     const uint32_t n_fast = const_ptr(d.meta)[slot].n_fast;
     const uint32_t tail_start = const_ptr(d.meta)[slot].tail_start;
     const uint32_t bias = const_ptr(d.meta)[slot].bias;
@@ -517,22 +544,45 @@ __device__ __forceinline__ void walk_twin(const TileDesc &d, uint32_t rel, bool 
         max_angle[b] = kMaxAngle;
         surface[b] = 0.0f;
     }
+    // One step with the IEEE divide (fast_k false) or the sector's divide.
+    auto single = [&](uint32_t k, bool fast_k) {
+        float h[NB];
+        load_pair<H>(raw, pov_bytes, pos0[k], h[0], h[1]);
+        if (BOTH) load_pair<H>(raw, pov_bytes, pos1[k], h[NB - 2], h[NB - 1]);
+        const float dk = dist[k];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const float dh = h[b] - ((b & 1) ? h0b : h0a);
+            const float q = (DIV != 0 && fast_k) ? step_div<DIV>(dh, dk, rcp[k], rcp_lo[k]) : dh / dk;
+            step_update(q - curv[k], wt[k], lanes, max_angle[b], surface[b]);
+        }
+    };
     uint32_t k = 0;
-    const uint32_t fast_end = n_fast - n_fast % UNROLL;
-    for (; k < fast_end; k += UNROLL) {
+    for (; k < head; ++k) single(k, false);
+    // The fast range [head, n_fast) in whole chunks, through the tables
+    // from step head on (f*), indexed from 0.
+    const uint32_t fast_end = (n_fast - head) - (n_fast - head) % UNROLL;
+    const TVS_CONST uint32_t *__restrict__ fpos0 = pos0 + head;
+    const TVS_CONST uint32_t *__restrict__ fpos1 = pos1 + head;
+    const TVS_CONST float *__restrict__ fdist = dist + head;
+    const TVS_CONST float *__restrict__ frcp = rcp + head;
+    const TVS_CONST float *__restrict__ frcp_lo = rcp_lo + head;
+    const TVS_CONST float *__restrict__ fcurv = curv + head;
+    const TVS_CONST float *__restrict__ fwt = wt + head;
+    for (uint32_t j = 0; j < fast_end; j += UNROLL) {
         float h[UNROLL][NB];
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
-            load_pair<H>(raw, pov_bytes, pos0[k + u], h[u][0], h[u][1]);
-            if (BOTH) load_pair<H>(raw, pov_bytes, pos1[k + u], h[u][NB - 2], h[u][NB - 1]);
+            load_pair<H>(raw, pov_bytes, fpos0[j + u], h[u][0], h[u][1]);
+            if (BOTH) load_pair<H>(raw, pov_bytes, fpos1[j + u], h[u][NB - 2], h[u][NB - 1]);
         }
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
-            const float dk = dist[k + u];
-            const float y_hi = rcp[k + u];
-            const float y_lo = rcp_lo[k + u];
-            const float c = curv[k + u];
-            const float w = wt[k + u];
+            const float dk = fdist[j + u];
+            const float y_hi = frcp[j + u];
+            const float y_lo = frcp_lo[j + u];
+            const float c = fcurv[j + u];
+            const float w = fwt[j + u];
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 const float dh = h[u][b] - ((b & 1) ? h0b : h0a);
@@ -541,19 +591,8 @@ __device__ __forceinline__ void walk_twin(const TileDesc &d, uint32_t rel, bool 
             }
         }
     }
-    for (; k < tail_start; ++k) {
-        float h[NB];
-        load_pair<H>(raw, pov_bytes, pos0[k], h[0], h[1]);
-        if (BOTH) load_pair<H>(raw, pov_bytes, pos1[k], h[NB - 2], h[NB - 1]);
-        const float dk = dist[k];
-        const bool fast_k = DIV != 0 && k < n_fast;
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const float dh = h[b] - ((b & 1) ? h0b : h0a);
-            const float q = fast_k ? step_div<DIV>(dh, dk, rcp[k], rcp_lo[k]) : dh / dk;
-            step_update(q - curv[k], wt[k], lanes, max_angle[b], surface[b]);
-        }
-    }
+    k += fast_end;
+    for (; k < tail_start; ++k) single(k, k < n_fast);
     if (active) {
         TVS_GLOBAL float *out =
             global_ptr(d.partial) + static_cast<size_t>(2u * rel + (BOTH ? 0u : (backward ? 1u : 0u))) * d.N + t0;
@@ -617,8 +656,10 @@ __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
     const TileDesc d = tiles[ti];
     const uint32_t local = PAIR ? (x - item_begin[ti]) >> 1 : x - item_begin[ti];
     const uint32_t pass = local / d.blocks_per_pass;     // sector (PAIR) or 2*sector + dir
-    const uint32_t ob = local - pass * d.blocks_per_pass;
     const uint32_t rel = PAIR ? pass : pass >> 1;
+    uint32_t ob = local - pass * d.blocks_per_pass;
+    if (d.order)   // directional launch: the sector's blocks in strip order
+        ob = const_ptr(d.order)[(d.first_slot + rel) * d.blocks_per_pass + ob];
     const bool backward = !PAIR && (pass & 1u) != 0;
     const int div = IEEE ? 0 : divide_of(d, rel);
     if (TWIN) {
@@ -672,14 +713,19 @@ __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
 }
 
 // One work item per block, or (PERSIST) a grid of a few blocks per CU
-// striding over all items.  With items_per_xcd != 0, consecutive block ids
-// (dispatched to the 8 XCDs round-robin) are remapped so each XCD takes a
-// contiguous run of items: one DEM region per L2.
+// striding over all items.  Consecutive block ids are dispatched to the 8
+// XCDs round-robin.  With items_per_xcd != 0 they are remapped so each XCD
+// takes a contiguous run of items: one DEM region per L2.  With pass_chunk
+// != 0 (one tile, pass_items items per pass) each pass's items are split in
+// 8 contiguous chunks of pass_chunk, one per XCD: every XCD takes an equal
+// share of every sector, and (with the directional block order) its share
+// is a strip along the sector's band direction.
 template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PERSIST, bool PAIR,
           bool TWIN, bool IEEE>
 __global__ __launch_bounds__(BLOCK) void walk_tiles_kernel(
     const TileDesc *__restrict__ tiles, const uint32_t *__restrict__ item_begin,
-    uint32_t n_tiles, uint32_t n_items, uint32_t items_per_xcd) {
+    uint32_t n_tiles, uint32_t n_items, uint32_t items_per_xcd, uint32_t pass_items,
+    uint32_t pass_chunk) {
     const uint32_t b = blockIdx.x;
     if (PERSIST) {
         for (uint32_t item = b; item < n_items; item += gridDim.x)
@@ -687,7 +733,16 @@ __global__ __launch_bounds__(BLOCK) void walk_tiles_kernel(
                                                                          n_tiles, item);
         return;
     }
-    const uint32_t item = items_per_xcd ? (b & 7u) * items_per_xcd + (b >> 3) : b;
+    uint32_t item;
+    if (pass_chunk) {
+        const uint32_t j = b >> 3;
+        const uint32_t pass = j / pass_chunk;
+        const uint32_t within = (b & 7u) * pass_chunk + (j - pass * pass_chunk);
+        if (within >= pass_items) return;
+        item = pass * pass_items + within;
+    } else {
+        item = items_per_xcd ? (b & 7u) * items_per_xcd + (b >> 3) : b;
+    }
     if (item >= n_items) return;
     walk_item<H, CAPTURE, BLOCK, UNROLL, PIPE, PAIR, TWIN, IEEE>(tiles, item_begin, n_tiles, item);
 }
@@ -766,7 +821,7 @@ __global__ __launch_bounds__(kBlock) void short_div_check_kernel(StepTables tab,
                                                             uint32_t *__restrict__ bad) {
     const uint32_t slot = first + blockIdx.x;
     const uint32_t n_fast = tab.meta[slot].n_fast;
-    const uint32_t k0 = blockIdx.y * kShortDivChunk;
+    const uint32_t k0 = tab.meta[slot].head + blockIdx.y * kShortDivChunk;
     if (k0 >= n_fast) return;   // uniform over the block
     const uint32_t nk = min(n_fast - k0, kShortDivChunk);
     __shared__ float sd[kShortDivChunk], shi[kShortDivChunk], slo[kShortDivChunk];
@@ -883,15 +938,22 @@ uint32_t persistent_blocks(uint32_t block) {
 template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PERSIST, bool PAIR,
           bool TWIN, bool IEEE>
 hipError_t launch_tiles_v(hipStream_t s, const TileDesc *tiles, const uint32_t *item_begin,
-                          uint32_t n_tiles, uint32_t n_items, bool xcd_contiguous) {
+                          uint32_t n_tiles, uint32_t n_items, bool xcd_contiguous,
+                          uint32_t pass_items) {
     if (PAIR && (n_items & 1u)) return hipErrorInvalidValue;
     const uint32_t items = PAIR ? n_items / 2u : n_items;
-    const uint32_t per_xcd = (xcd_contiguous && !PERSIST) ? (items + 7u) / 8u : 0u;
-    uint32_t grid = per_xcd ? per_xcd * 8u : items;
+    // Split passes: one tile whose items are whole passes.
+    const bool split = !PERSIST && xcd_contiguous && n_tiles == 1u && pass_items != 0u &&
+                       items % pass_items == 0u;
+    const uint32_t chunk = split ? (pass_items + 7u) / 8u : 0u;
+    const uint32_t per_xcd = (xcd_contiguous && !PERSIST && !split) ? (items + 7u) / 8u : 0u;
+    uint64_t grid = split ? 8ull * chunk * (items / pass_items) : per_xcd ? per_xcd * 8ull : items;
     if (PERSIST) grid = std::min(items, persistent_blocks(BLOCK));
+    if (grid > 0xffffffffull) return hipErrorInvalidValue;
     hipLaunchKernelGGL((walk_tiles_kernel<H, CAPTURE, BLOCK, UNROLL, PIPE, PERSIST, PAIR, TWIN,
                                           IEEE>),
-                       dim3(grid), dim3(BLOCK), 0, s, tiles, item_begin, n_tiles, items, per_xcd);
+                       dim3(static_cast<uint32_t>(grid)), dim3(BLOCK), 0, s, tiles, item_begin,
+                       n_tiles, items, per_xcd, split ? pass_items : 0u, chunk);
     return hipGetLastError();
 }
 
@@ -899,27 +961,28 @@ template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PERSI
           bool TWIN>
 hipError_t launch_tiles_t(hipStream_t s, const TileDesc *tiles, const uint32_t *item_begin,
                           uint32_t n_tiles, uint32_t n_items, bool xcd_contiguous,
-                          bool ieee_only) {
+                          bool ieee_only, uint32_t pass_items) {
     return ieee_only
                ? launch_tiles_v<H, CAPTURE, BLOCK, UNROLL, PIPE, PERSIST, PAIR, TWIN, true>(
-                     s, tiles, item_begin, n_tiles, n_items, xcd_contiguous)
+                     s, tiles, item_begin, n_tiles, n_items, xcd_contiguous, pass_items)
                : launch_tiles_v<H, CAPTURE, BLOCK, UNROLL, PIPE, PERSIST, PAIR, TWIN, false>(
-                     s, tiles, item_begin, n_tiles, n_items, xcd_contiguous);
+                     s, tiles, item_begin, n_tiles, n_items, xcd_contiguous, pass_items);
 }
 
 template <typename H>
 hipError_t launch_tiles_h(hipStream_t s, const TileDesc *tiles, const uint32_t *item_begin,
                           uint32_t n_tiles, uint32_t n_items, bool capture, bool ieee_only,
-                          const WalkConfig &cfg) {
+                          const WalkConfig &cfg, uint32_t pass_items) {
     if (capture)   // ring capture is a checking path: one shape (capture_walk_config)
         return launch_tiles_t<H, true, 256, 8, false, false, false, false>(
-            s, tiles, item_begin, n_tiles, n_items, cfg.xcd_contiguous, ieee_only);
+            s, tiles, item_begin, n_tiles, n_items, cfg.xcd_contiguous, ieee_only, 0u);
+    const uint32_t split_items = cfg.directional ? pass_items : 0u;
 #define TVS_SHAPE(B, U, P, S, D, W)                                                        \
     if (cfg.block == B && cfg.unroll == U && cfg.pipeline == P && cfg.persistent == S &&   \
         cfg.pair == D && cfg.twin == W)                                                    \
         return launch_tiles_t<H, false, B, U, P, S, D, W>(s, tiles, item_begin, n_tiles,   \
                                                           n_items, cfg.xcd_contiguous,     \
-                                                          ieee_only);
+                                                          ieee_only, split_items);
     TVS_WALK_SHAPES(TVS_SHAPE)
 #undef TVS_SHAPE
     return hipErrorInvalidValue;
@@ -952,12 +1015,12 @@ bool walk_shape_supported(const WalkConfig &cfg) {
 
 hipError_t launch_walk_tiles(hipStream_t s, const TileDesc *tiles, const uint32_t *item_begin,
                              uint32_t n_tiles, uint32_t n_items, bool i16, bool capture,
-                             bool ieee_only, const WalkConfig &cfg) {
+                             bool ieee_only, const WalkConfig &cfg, uint32_t pass_items) {
     if (n_items == 0) return hipSuccess;
     return i16 ? launch_tiles_h<int16_t>(s, tiles, item_begin, n_tiles, n_items, capture,
-                                         ieee_only, cfg)
+                                         ieee_only, cfg, pass_items)
                : launch_tiles_h<float>(s, tiles, item_begin, n_tiles, n_items, capture,
-                                       ieee_only, cfg);
+                                       ieee_only, cfg, pass_items);
 }
 
 hipError_t launch_short_div_check(hipStream_t s, const StepTables &tab, uint32_t first,

@@ -108,6 +108,7 @@ class WalkShape(ctypes.Structure):
     def as_dict(self) -> dict:
         return {"block": self.block, "unroll": self.unroll,
                 "xcd_contiguous": bool(self.xcd_contiguous),
+                "directional": self.xcd_contiguous == 2,
                 "pipeline": bool(self.pipeline), "persistent": bool(self.persistent),
                 "both_directions": bool(self.both_directions),
                 "two_points": bool(self.two_points), "ms": self.ms}
@@ -404,7 +405,7 @@ class Engine:
         self._c(load().tvs_set_short_divide(self._h, int(enable)))
 
     def short_divide(self) -> tuple[bool, int]:
-        """(enabled, sectors verified for the three-operation divide)."""
+        """(enabled, sectors verified for the two-operation divide)."""
         en, n = ctypes.c_uint32(0), ctypes.c_uint32(0)
         self._c(load().tvs_get_short_divide(self._h, ctypes.byref(en), ctypes.byref(n)))
         return bool(en.value), int(n.value)
