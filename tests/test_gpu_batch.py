@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import tvs_amd
-from tvs_amd import demgen
+from tvs_amd import demgen, _lib
 
 pytestmark = pytest.mark.gpu
 THREADS = min(16, os.cpu_count() or 1)
@@ -128,7 +128,10 @@ def test_autotune_keeps_results_exact(tiles_and_truth):
         before = e.surfaces()
         best, table = e.autotune(0, 30, reps=1)
         print("autotune", best, flush=True)
-        assert len(table) == 2 * 22 + 5 and best["ms"] > 0   # persistent shapes once
+        # every compiled shape in both item orders, the two-point shapes also
+        # in the directional one, persistent shapes once
+        assert len(table) == _lib.load().tvs_walk_shape_count() == 2 * 22 + 5 + 6
+        assert best["ms"] > 0
         np.testing.assert_array_equal(e.surfaces(), before)   # surfaces preserved
         assert e.walk_shape()["block"] == best["block"]
         e.reset_surfaces()
@@ -149,7 +152,8 @@ def test_batch_autotune_keeps_results_exact(tiles_and_truth):
         before = [b.surfaces(i) for i in range(len(tiles))]
         best, table = b.autotune(reps=1)
         print("batch autotune", best, flush=True)
-        assert len(table) == 2 * 22 + 5 and best["ms"] > 0
+        assert len(table) == _lib.load().tvs_walk_shape_count() == 2 * 22 + 5 + 6
+        assert best["ms"] > 0
         for i in range(len(tiles)):
             np.testing.assert_array_equal(b.surfaces(i), before[i])
         b.reset_surfaces()
