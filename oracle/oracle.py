@@ -19,7 +19,9 @@ from dataclasses import dataclass
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libtvs_oracle.so")
+# TVS_ORACLE_LIB names another build of the same source (the sanitized one,
+# or a variant with a planted mistake that a test expects to be caught).
+LIB_PATH = os.environ.get("TVS_ORACLE_LIB") or os.path.join(HERE, "libtvs_oracle.so")
 
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 
@@ -59,7 +61,7 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        if not os.path.exists(LIB_PATH) and "TVS_ORACLE_LIB" not in os.environ:
             build()
         L = ctypes.CDLL(LIB_PATH)
         vp = ctypes.c_void_p

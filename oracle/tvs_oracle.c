@@ -500,7 +500,14 @@ static double band_f64(const tvso_constants *c, const float *elev,
         double angd = (((double)elev[id] - h0d) / dd) -
                       (dd / (double)TVSO_EARTH_RADIUS_SQUARED);
         int vis = ang > mx;
+#ifdef TVSO_PLANTED_F32_VISIBILITY
+        /* A planted mistake, compiled only by tests/test_oracle_sanitized.py
+         * to show tests/test_oracle.py catches it: the f64 walk takes its
+         * visibility from the f32 angle. */
+        int visd = vis;
+#else
         int visd = angd > mxd;
+#endif
         f += (uint64_t)(vis != visd);
         if (visd) surf += dd * 0.0174533;
         mx = fmaxf(ang, mx);

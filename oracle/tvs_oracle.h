@@ -13,8 +13,9 @@
  *       -> crates/total-viewsheds/src/
  *   kernel.rs -> crates/shader/src/
  *
- * Pinning (what checks each function beyond retyping its formula) is listed
- * in oracle/README.md.
+ * What pins each function beyond retyping its formula (the reference's own
+ * golden vectors, closed forms, brute force, constructed cases) is checked
+ * by tests/test_oracle.py.
  */
 #ifndef TVS_ORACLE_H
 #define TVS_ORACLE_H

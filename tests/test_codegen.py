@@ -151,3 +151,43 @@ def test_no_packed_fp32_and_no_spills(asm):
     assert not re.search(r"\bv_pk_(add|mul|fma)_f32\b", asm)
     sizes = re.findall(r"\.private_segment_fixed_size:\s+(\d+)", asm)
     assert sizes and all(int(x) == 0 for x in sizes)
+
+
+DEVICE_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+                "-fhip-fp32-correctly-rounded-divide-sqrt", "--cuda-device-only", "-S"]
+NO_PACKED = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def test_no_packed_fp32_in_band_kernels(tmp_path):
+    """bands.hip (sort, selection, step tables) compiled as the library is:
+    no packed-FP32 instruction and no scratch."""
+    out = tmp_path / "bands.s"
+    subprocess.run([HIPCC, *DEVICE_FLAGS, *NO_PACKED, "-o", str(out),
+                    os.path.join(ROOT, "total-viewsheds_amd", "csrc", "bands.hip")],
+                   check=True, capture_output=True)
+    text = out.read_text()
+    assert not re.search(r"\bv_pk_(add|mul|fma)_f32\b", text)
+    sizes = re.findall(r"\.private_segment_fixed_size:\s+(\d+)", text)
+    assert sizes and all(int(x) == 0 for x in sizes)
+
+
+def test_packed_fp32_flag_is_effective(tmp_path):
+    """The flag the library is built with takes effect on the device pass:
+    a float2 add/multiply kernel uses v_pk_*_f32 without it and none with
+    it (so the absence checks above are not vacuous)."""
+    src = tmp_path / "pk.hip"
+    src.write_text(
+        "#include <hip/hip_runtime.h>\n"
+        "__global__ void k(float2 *a, const float2 *b, const float2 *c, int n) {\n"
+        "  int i = blockIdx.x * blockDim.x + threadIdx.x;\n"
+        "  if (i < n) { float2 x = b[i], y = c[i];\n"
+        "    a[i] = make_float2(x.x + y.x, x.y + y.y);\n"
+        "    a[i + n] = make_float2(x.x * y.x, x.y * y.y); }\n"
+        "}\n")
+    counts = []
+    for extra in ([], NO_PACKED):
+        out = tmp_path / f"pk{len(extra)}.s"
+        subprocess.run([HIPCC, *DEVICE_FLAGS, *extra, "-o", str(out), str(src)], check=True,
+                       capture_output=True)
+        counts.append(len(re.findall(r"\bv_pk_(?:add|mul|fma)_f32\b", out.read_text())))
+    assert counts[0] > 0 and counts[1] == 0, counts

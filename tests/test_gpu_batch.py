@@ -190,3 +190,61 @@ def test_batch_ieee_divide_tiles(oracle, tiles_and_truth, tiny_tiles):
         for i in range(len(tiles)):
             np.testing.assert_array_equal(b.surfaces(i).view(np.uint32),
                                           want[i].view(np.uint32), err_msg=f"tile {i}")
+
+
+def test_batch_mutators_wait_for_the_running_batch(oracle, tiles_and_truth):
+    """tvs_batch_run is asynchronous: an upload (or any other change) issued
+    right after it, with no wait in between, must not reach the heights the
+    running walk reads.  Run the graph, upload new heights to every tile at
+    once, run again: the first results are the old heights' (read after the
+    second run is queued), the second the new heights'."""
+    tiles, truth = tiles_and_truth
+    rng = np.random.default_rng(21)
+    new = []
+    want_new = []
+    for t in tiles:
+        h = rng.permutation(t.elevations)
+        new.append(h)
+        dem = oracle.dem_new(t.width, t.spacing, t.max_line_of_sight)
+        cum, _, _ = oracle.compute_run(dem, h.astype(np.float32), t.observer_height,
+                                       t.rings_per_km, rings=False, n_threads=THREADS)
+        want_new.append(cum)
+    print("batch: upload right after an asynchronous run", flush=True)
+    with tvs_amd.Batch([(t.width, t.spacing, t.max_line_of_sight, t.rings_per_km,
+                         t.observer_height) for t in tiles]) as b:
+        for i, t in enumerate(tiles):
+            b.upload(i, t.elevations)
+        b.compute_bands()
+        b.set_walk_shape(256, 8)          # one band per lane: the slowest shape
+        b.run(True)                       # no sync before the uploads
+        for i, h in enumerate(new):
+            b.upload(i, h)
+        first = [b.surfaces(i) for i in range(len(tiles))]
+        b.reset_surfaces()
+        b.run(True)
+        b.set_walk_shape(64, 8, both_directions=True, two_points=True)   # no sync either
+        second = [b.surfaces(i) for i in range(len(tiles))]
+        for i in range(len(tiles)):
+            np.testing.assert_array_equal(first[i], truth[i], err_msg=f"old heights, tile {i}")
+            np.testing.assert_array_equal(second[i], want_new[i], err_msg=f"new heights, tile {i}")
+
+
+def test_autotune_leaves_ring_capture_alone(tiles_and_truth):
+    """With ring capture on, the tuner's timing runs record no rings (every
+    shape is timed as itself, not as the capture kernel) and the rings and
+    overflow count of the caller's last run stay readable and unchanged."""
+    tiles, _ = tiles_and_truth
+    t = tiles[0]
+    with tvs_amd.Engine(t.width, t.spacing, t.max_line_of_sight, 0.5,
+                        t.observer_height, 1) as e:
+        e.upload(t.elevations)
+        e.compute_bands(0, 180)
+        e.run(17, 1, "batched")
+        rings_before = e.rings(0)
+        overflow_before = e.ring_overflow()
+        assert overflow_before > 0          # R is small: some bands overflow
+        best, table = e.autotune(0, 12, reps=1)
+        print("autotune with capture on", best, flush=True)
+        np.testing.assert_array_equal(e.rings(0), rings_before)
+        assert e.ring_overflow() == overflow_before
+        assert len(table) == _lib.load().tvs_walk_shape_count() and best["ms"] > 0

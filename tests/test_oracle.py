@@ -247,3 +247,90 @@ def test_f64_companion(oracle):
     flat = np.full(61 * 61, 100.0, np.float32)
     _, flips_flat, _ = oracle.walk_sectors_f64(dem, flat, 1.65, deltas, dists)
     assert flips_flat == 0
+
+
+def test_f64_flat_dem_closed_form(oracle):
+    """On a flat DEM the f64 angle -h/d - d/12.742e6 (h = the observer
+    height, exact in double) grows strictly with d for every d far below
+    sqrt(h * 12.742e6) ~ 4585 grid steps, and distinct f32 distances differ
+    by far more than double rounding can hide.  So a step is visible in f64
+    exactly when its distance exceeds every distance before it in the band
+    (a zero distance gives -inf and is never visible), and each TVS value is
+    the in-order double sum over sectors of (s + s), s the sum of
+    d * 0.0174533 over those record-breaking steps -- from band data alone,
+    with no angle, division or max-scan computed."""
+    w, los = 61, 20
+    dem = oracle.dem_new(w, 1.0, los)
+    elev = np.full(w * w, 250.0, np.float32)
+    sectors = list(range(0, 180, 7))
+    deltas, dists = oracle.band_data_all(dem, sectors)
+    want = 0.0
+    for row in dists:
+        surf, longest = 0.0, 0.0
+        for d in row.astype(np.float64):
+            if d > longest:
+                surf += d * 0.0174533
+            longest = max(longest, d)
+        want += surf
+        want += surf
+    tvs64, flips, _ = oracle.walk_sectors_f64(dem, elev, 1.65, deltas, dists, n_threads=4)
+    assert flips == 0
+    assert np.all(tvs64 == want), (tvs64[:3], want)
+
+
+def test_f64_constructed_visibility_flip(oracle):
+    """A band built so that f32 and f64 provably see differently.
+
+    9x9 DEM (L = 3, observers at x, y in 3..5), one sector of hand-made band
+    data: deltas (1, 1, 1) (forward = east along the row), distances
+    (2, 1, 3), observer height 0, heights 0 except row 3: x = 4 is 2000 m,
+    x = 5 is 1000 m.  With E = 12742000:
+
+    * observer (3, 3), forward: step 1 sees 2000/2 - 2/E; step 2 sees
+      1000/1 - 1/E, larger by 1/E = 7.8e-8 in f64 (visible) but rounded to
+      the same f32 1000.0 as step 1 (ulp 6.1e-5), and `>` is strict, so
+      f32 hides it: the only flip.  Step 3 (0 m at d 3) is hidden.
+    * observer (4, 3) (h0 2000): forward -500, -2000, -666.7: step 1 only;
+      backward -1000, -2000, -666.7: steps 1 and 3.
+    * observer (5, 3) (h0 1000): forward -500, -1000, -333.3: steps 1, 3;
+      backward 500, -1000, -333.3: step 1.
+    * every other band runs over zero heights from h0 = 0: -2/E, -1/E,
+      -3/E, steps 1 and 2 visible in both precisions (f32 keeps full
+      relative precision there).
+
+    So flips = 1 of 54 steps, and the f64 surfaces are the double sums of
+    d * 0.0174533 over those steps (forward then backward); the f32 walk
+    gives the same sets except observer (3, 3) forward, which loses step 2.
+    A f64 walk taking its visibility from the f32 angle fails both."""
+    w = 9
+    dem = oracle.dem_new(w, 1.0, 3)
+    assert (dem.max_los_as_points, dem.tvs_width, dem.n_deltas) == (3, 3, 3)
+    elev = np.zeros(w * w, np.float32)
+    elev[3 * w + 4] = 2000.0
+    elev[3 * w + 5] = 1000.0
+    deltas = np.array([[1, 1, 1]], np.int32)
+    dists = np.array([[2.0, 1.0, 3.0]], np.float32)
+    visible = {   # observer t -> (forward steps, backward steps), f64
+        0: ((0, 1), (0, 1)), 1: ((0,), (0, 2)), 2: ((0, 2), (0,)),
+    }
+    want64 = np.zeros(dem.computable_points_count)
+    want32 = np.zeros(dem.computable_points_count, np.float32)
+    for t in range(dem.computable_points_count):
+        fwd, bwd = visible.get(t, ((0, 1), (0, 1)))
+        for band in (fwd, bwd):
+            s = 0.0
+            for i in band:
+                s += float(dists[0, i]) * 0.0174533
+            want64[t] += s
+        for b, band in enumerate((fwd, bwd)):
+            f32_band = (0,) if (t == 0 and b == 0) else band
+            s32 = np.float32(0.0)
+            for i in f32_band:
+                s32 = np.float32(s32 + np.float32(dists[0, i] * np.float32(0.0174533)))
+            want32[t] = np.float32(want32[t] + s32)
+    tvs64, flips, steps = oracle.walk_sectors_f64(dem, elev, 0.0, deltas, dists)
+    assert steps == 54
+    assert flips == 1
+    assert np.array_equal(tvs64, want64), (tvs64, want64)
+    cum, _ = oracle.walk_sectors(dem, elev, 0.0, deltas, dists)
+    assert np.array_equal(cum, want32), (cum, want32)

@@ -758,7 +758,12 @@ static tvs_status begin_capture(tvs_ctx *c, uint32_t capture) {
     return TVS_OK;
 }
 
-tvs_status tvs_run_sectors(tvs_ctx *c, uint32_t first, uint32_t count, tvs_method method) {
+}  // extern "C"
+
+namespace tvs {
+
+tvs_status run_sectors(tvs_ctx *c, uint32_t first, uint32_t count, tvs_method method,
+                       bool keep_rings) {
     if (!c) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null context");
     if (first >= tvs::kSectorSteps || count > tvs::kSectorSteps - first)
         return fail(c, TVS_ERR_INVALID_ARGUMENT, "sector range outside 0..180");
@@ -769,8 +774,8 @@ tvs_status tvs_run_sectors(tvs_ctx *c, uint32_t first, uint32_t count, tvs_metho
         if (!c->band_ready[s])
             return fail(c, TVS_ERR_STATE, "sector " + std::to_string(s) + " has no band data");
     const uint32_t N = c->geo.computable_points_count;
-    const uint32_t capture = std::min(c->params.ring_capture_sectors, count);
-    tvs_status st = begin_capture(c, capture);
+    const uint32_t capture = keep_rings ? 0u : std::min(c->params.ring_capture_sectors, count);
+    tvs_status st = keep_rings ? TVS_OK : begin_capture(c, capture);
     if (st != TVS_OK) return st;
     c->timing = tvs_timing{0.0f, 0.0f, c->timing.bands_ms, 0u,
                            static_cast<uint64_t>(count) * 2ull * N * c->geo.band_deltas_size};
@@ -830,8 +835,16 @@ tvs_status tvs_run_sectors(tvs_ctx *c, uint32_t first, uint32_t count, tvs_metho
         c->timing.walk_launches = 1;
     }
     ++c->walk_runs;
-    c->ring_valid = capture;
+    if (!keep_rings) c->ring_valid = capture;
     return TVS_OK;
+}
+
+}  // namespace tvs
+
+extern "C" {
+
+tvs_status tvs_run_sectors(tvs_ctx *c, uint32_t first, uint32_t count, tvs_method method) {
+    return tvs::run_sectors(c, first, count, method, false);
 }
 
 tvs_status tvs_run_sector_with_band(tvs_ctx *c, const int32_t *deltas, const float *dists,

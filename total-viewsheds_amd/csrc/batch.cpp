@@ -68,7 +68,17 @@ tvs_status from_tile(tvs_batch *b, uint32_t i, tvs_status st) {
     return st;
 }
 
+// A run is asynchronous on the batch's stream and reads every tile's
+// heights, step tables, divide flags and descriptors, and replays the graph
+// exec: each call that changes one of them waits for the stream first.
+tvs_status quiesce(tvs_batch *b) {
+    TVS_BHIP(b, hipStreamSynchronize(b->stream), "wait for the batch's last run");
+    return TVS_OK;
+}
+
 void drop_graph(tvs_batch *b) {
+    // The exec may still be running: destroy it only once the stream is idle.
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
@@ -230,6 +240,8 @@ tvs_status tvs_batch_tile_geometry(const tvs_batch *b, uint32_t tile, tvs_geomet
 tvs_status tvs_batch_upload_i16(tvs_batch *b, uint32_t tile, const int16_t *elevations,
                                 size_t count) {
     if (!b || tile >= b->tiles.size()) return bfail(b, TVS_ERR_INVALID_ARGUMENT, "tile index");
+    const tvs_status q = quiesce(b);
+    if (q != TVS_OK) return q;
     drop_graph(b);
     return from_tile(b, tile, tvs_upload_dem_i16(b->tiles[tile], elevations, count));
 }
@@ -237,12 +249,16 @@ tvs_status tvs_batch_upload_i16(tvs_batch *b, uint32_t tile, const int16_t *elev
 tvs_status tvs_batch_upload_f32(tvs_batch *b, uint32_t tile, const float *elevations,
                                 size_t count) {
     if (!b || tile >= b->tiles.size()) return bfail(b, TVS_ERR_INVALID_ARGUMENT, "tile index");
+    const tvs_status q = quiesce(b);
+    if (q != TVS_OK) return q;
     drop_graph(b);
     return from_tile(b, tile, tvs_upload_dem_f32(b->tiles[tile], elevations, count));
 }
 
 tvs_status tvs_batch_compute_bands(tvs_batch *b) {
     if (!b) return bfail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null batch");
+    const tvs_status q = quiesce(b);
+    if (q != TVS_OK) return q;
     float ms = 0.0f;
     for (uint32_t i = 0; i < b->tiles.size(); ++i) {
         if (b->owner[i] != i) continue;
@@ -274,6 +290,8 @@ tvs_status tvs_batch_set_walk_shape(tvs_batch *b, const tvs_walk_shape *shape) {
                         shape->xcd_contiguous == 2u};
     if (!tvs::walk_shape_supported(cfg))
         return bfail(b, TVS_ERR_INVALID_ARGUMENT, "walk shape not compiled");
+    const tvs_status q = quiesce(b);
+    if (q != TVS_OK) return q;
     b->cfg = cfg;
     b->cfg_set = true;
     drop_graph(b);
@@ -286,7 +304,8 @@ tvs_status tvs_batch_autotune(tvs_batch *b, uint32_t reps, tvs_walk_shape *resul
     if (reps == 0) reps = 1;
     // Only the walk launch is timed: it writes the per-sector band surfaces
     // (scratch), never the cumulative surfaces, so the caller's totals stay.
-    TVS_BHIP(b, hipStreamSynchronize(b->stream), "sync");
+    const tvs_status q = quiesce(b);
+    if (q != TVS_OK) return q;
     const tvs::WalkConfig keep = b->cfg;
     const bool keep_set = b->cfg_set;
     b->cfg_set = true;   // prepare() must use the shape under test
@@ -341,6 +360,8 @@ tvs_status tvs_batch_autotune(tvs_batch *b, uint32_t reps, tvs_walk_shape *resul
 
 tvs_status tvs_batch_set_short_divide(tvs_batch *b, uint32_t enable) {
     if (!b) return bfail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null batch");
+    const tvs_status q = quiesce(b);
+    if (q != TVS_OK) return q;
     for (uint32_t i = 0; i < b->tiles.size(); ++i) {
         const tvs_status st = tvs_set_short_divide(b->tiles[i], enable);
         if (st != TVS_OK) return from_tile(b, i, st);

@@ -322,7 +322,12 @@ hipError_t launch_ring_offsets(hipStream_t s, const uint32_t *counts, uint32_t n
 hipError_t launch_widen_i16(hipStream_t s, const int16_t *in, float *out,
                             size_t n);
 
-// ---- host side of a batched launch (api.cpp) ----
+// ---- host side of a run (api.cpp) ----
+// tvs_run_sectors; with keep_rings the run records no rings and leaves the
+// ring buffer, the overflow counter and the captured-slot count untouched
+// (the tuner's timing runs).
+tvs_status run_sectors(tvs_ctx *c, uint32_t first, uint32_t count, tvs_method method,
+                       bool keep_rings);
 tvs_status validate_slots(tvs_ctx *c, uint32_t first, uint32_t count);
 tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t capture,
                         const WalkConfig &cfg, TileDesc *out);

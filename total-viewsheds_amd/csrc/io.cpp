@@ -92,19 +92,22 @@ std::string cache_file(const char *state_dir, uint32_t width, const char *kind,
 }
 
 // ---- PNG (stored deflate blocks; greyscale 8-bit) ----
-uint32_t crc_table[256];
-bool crc_ready = false;
+struct CrcTable {
+    uint32_t v[256];
+    CrcTable() {
+        for (uint32_t i = 0; i < 256; ++i) {
+            uint32_t x = i;
+            for (int k = 0; k < 8; ++k) x = (x & 1u) ? 0xedb88320u ^ (x >> 1) : x >> 1;
+            v[i] = x;
+        }
+    }
+};
 
 uint32_t crc32(const unsigned char *p, size_t n, uint32_t c = 0xffffffffu) {
-    if (!crc_ready) {
-        for (uint32_t i = 0; i < 256; ++i) {
-            uint32_t v = i;
-            for (int k = 0; k < 8; ++k) v = (v & 1u) ? 0xedb88320u ^ (v >> 1) : v >> 1;
-            crc_table[i] = v;
-        }
-        crc_ready = true;
-    }
-    for (size_t i = 0; i < n; ++i) c = crc_table[(c ^ p[i]) & 0xffu] ^ (c >> 8);
+    // A function-local static is initialised once, thread-safely, on first
+    // use: PNGs may be written from several threads at once.
+    static const CrcTable table;
+    for (size_t i = 0; i < n; ++i) c = table.v[(c ^ p[i]) & 0xffu] ^ (c >> 8);
     return c;
 }
 

@@ -5,7 +5,8 @@
 // round-robin item mapping, a persistent grid, both bands of a point per
 // lane, two points per lane) are timed on the context's own DEM and band data; the fastest
 // becomes the context's shape.  The cumulative surfaces are saved and
-// restored around the timing runs.
+// restored around the timing runs; the timing runs record no rings and leave
+// the ring records and overflow count of the caller's last run in place.
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -162,10 +163,13 @@ tvs_status tvs_autotune(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t rep
     tvs_status st = TVS_OK;
     for (const tvs::WalkConfig &cfg : tvs::compiled_walk_shapes()) {
         c->walk_cfg = cfg;
-        st = tvs_run_sectors(c, first, count, TVS_METHOD_BATCHED);   // warm-up
+        // Timing runs record no rings (with capture on, every shape would
+        // time the one capture kernel) and leave the caller's rings as they
+        // are.
+        st = tvs::run_sectors(c, first, count, TVS_METHOD_BATCHED, true);   // warm-up
         float total = 0.0f;
         for (uint32_t r = 0; r < reps && st == TVS_OK; ++r) {
-            st = tvs_run_sectors(c, first, count, TVS_METHOD_BATCHED);
+            st = tvs::run_sectors(c, first, count, TVS_METHOD_BATCHED, true);
             tvs_timing t;
             if (st == TVS_OK) st = tvs_get_timing(c, &t);
             total += t.walk_ms;
