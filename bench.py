@@ -34,8 +34,15 @@ sys.path.insert(0, os.path.join(ROOT, "total-viewsheds_amd"))
 METRIC = ("band-of-sight point-steps/s on 1201² DEM; batched vs plain at "
           "≤1e-4 TVS rel err")
 HBM_PEAK_GBPS = 8000.0
-# VALU lane-operations per second: 256 CUs x 64 lanes x 2.4 GHz.
-VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
+# Vector-instruction issue ceiling of MI355X: 256 CUs x 4 SIMDs, each SIMD
+# issuing at most two wave64 VALU instructions per four clocks (the
+# SQ_ACTIVE_INST_VALU2 counter counts the quad-cycles in which a SIMD issued
+# two), at the 2.4 GHz peak engine clock; in lane-operations (64 per
+# wave64 instruction): 78.6e12/s.  A single-issue stream (one wave64 VALU
+# instruction per four clocks) reaches half of it.
+SIMDS = 256 * 4
+CLOCK_HZ = 2.4e9
+VALU_ISSUE_PEAK_LANE_OPS = SIMDS * (2.0 / 4.0) * CLOCK_HZ * 64
 # Vector instructions per real band-step of the batched walk's unrolled
 # loop: sub, the divide (4 operations, or 2 with the verified short divide),
 # sub, cmpx, masked add, masked move, plus one conversion when two points
@@ -70,7 +77,7 @@ def time_method(e, method: str, steps: int, warmup: int):
     return wall * 1000.0 / steps, float(np.mean(kernel_ms)), e.timing()
 
 
-def cpu_baseline(wl, sectors: int, threads: int):
+def cpu_baseline(wl, sectors: int, threads: int, keep_surfaces: bool = False):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from concurrent.futures import ThreadPoolExecutor
@@ -87,7 +94,8 @@ def cpu_baseline(wl, sectors: int, threads: int):
     bands_s = time.perf_counter() - t0
     elev = wl.elevations.astype(np.float32)
     t0 = time.perf_counter()
-    O.walk_sectors(dem, elev, wl.observer_height, deltas, dists, n_threads=threads)
+    surfaces, _ = O.walk_sectors(dem, elev, wl.observer_height, deltas, dists,
+                                 n_threads=threads)
     dt = time.perf_counter() - t0
     steps = len(picked) * 2 * dem.computable_points_count * dem.n_deltas
     cpu = platform.processor() or platform.machine()
@@ -99,7 +107,7 @@ def cpu_baseline(wl, sectors: int, threads: int):
                     break
     except OSError:
         pass
-    return {"value": steps / dt, "unit": "point-steps/s", "cores": threads,
+    res = {"value": steps / dt, "unit": "point-steps/s", "cores": threads,
             "kind": "port",
             "bands_ms_per_sector": bands_s * 1000.0 / len(picked),
             "bands_s": bands_s, "walk_s": dt,
@@ -110,6 +118,7 @@ def cpu_baseline(wl, sectors: int, threads: int):
                        f"restatement of the reference CPU loop, {threads} thread(s), "
                        f"{dt:.1f} s on {cpu}; band data (axes sorts + compile) timed "
                        f"separately, {threads} thread(s)")}
+    return (res, surfaces) if keep_surfaces else res
 
 
 def measure_large(steps: int, warmup: int, cpu: bool = False) -> dict:
@@ -143,7 +152,7 @@ def measure_large(steps: int, warmup: int, cpu: bool = False) -> dict:
            "point_steps_per_step": ps, "nominal_point_steps_per_step": nominal,
            "bands_ms": bands_ms,
            "ring_export_ms_one_sector": ring_ms, "ring_events_one_sector": int(events.size),
-           "achieved_GBps": ps * 2 / (kernel_ms / 1000.0) / 1e9,
+           "gather_GBps_l2_served": ps * 2 / (kernel_ms / 1000.0) / 1e9,
            "walk_shape": {k: shape[k] for k in ("block", "unroll", "xcd_contiguous", "directional", "pipeline",
                                                 "persistent", "both_directions", "two_points")}}
     if cpu:
@@ -155,18 +164,25 @@ def measure_large(steps: int, warmup: int, cpu: bool = False) -> dict:
     return res
 
 
-def cpu_tiles(tiles, sectors: int, threads: int) -> dict:
-    """The oracle walk over `sectors` sectors of every tile, summed."""
-    parts = [cpu_baseline(t, sectors, threads) for t in tiles]
+def cpu_tiles(tiles, sectors: int, threads: int, gpu_surfaces=None) -> dict:
+    """The oracle walk over `sectors` sectors of every tile, summed; with
+    `gpu_surfaces` (180 sectors only) each tile's TVS is compared with it."""
+    runs = [cpu_baseline(t, sectors, threads, keep_surfaces=True) for t in tiles]
+    parts = [r[0] for r in runs]
     walk_s = sum(p["walk_s"] for p in parts)
     steps = sum(p["value"] * p["walk_s"] for p in parts)
-    return {"value": steps / walk_s, "unit": "point-steps/s", "cores": threads, "kind": "port",
+    res = {"value": steps / walk_s, "unit": "point-steps/s", "cores": threads, "kind": "port",
             "bands_s": sum(p["bands_s"] for p in parts), "walk_s": walk_s,
             "walk_s_180_sectors": sum(p["walk_s_180_sectors"] for p in parts),
             "sample": (f"{'all 180 sectors' if sectors >= 180 else f'{sectors} of 180 sectors (every {180 // sectors}th)'}"
                        f" of each of the {len(tiles)} tiles of tiles-64, walk only (every "
                        f"step, zero tail included), oracle C restatement of the reference CPU "
                        f"loop, {threads} thread(s), {walk_s:.1f} s; band data timed separately")}
+    if gpu_surfaces is not None and sectors >= 180:
+        res["tvs_bit_exact_vs_oracle"] = all(
+            np.array_equal(g.view(np.uint32), r[1].view(np.uint32))
+            for g, r in zip(gpu_surfaces, runs))
+    return res
 
 
 def measure_tiles(steps: int, warmup: int, cpu: bool = False) -> dict:
@@ -199,8 +215,10 @@ def measure_tiles(steps: int, warmup: int, cpu: bool = False) -> dict:
         graph_ms = [b.timing().walk_ms]        # the last replay (walk + reduction)
         nominal = b.timing().point_steps
         ps = sum(180 * 2 * g.computable_points_count * g.real_steps for g in b.geometry)
+        b.reset_surfaces()
         b.run(False)                         # the launch pair without the graph
         tm = b.timing()
+        surfaces = [b.surfaces(i) for i in range(len(tiles))]
     log(f"tiles-64: {ms:.2f} ms/step, {ps / ms * 1e3:.3e} executed point-steps/s")
     res = {"workload": "tiles-64", "value": ps / (ms / 1000.0), "unit": "point-steps/s",
             "nominal_value": nominal / (ms / 1000.0),
@@ -218,7 +236,7 @@ def measure_tiles(steps: int, warmup: int, cpu: bool = False) -> dict:
         res["cpu_one_core"] = cpu_tiles(tiles, 9, 1)
         cores = min(16, len(os.sched_getaffinity(0)))
         log(f"tiles-64: CPU oracle walk, {cores} threads, all 180 sectors of every tile")
-        res["cpu_all_cores"] = cpu_tiles(tiles, 180, cores)
+        res["cpu_all_cores"] = cpu_tiles(tiles, 180, cores, surfaces)
     return res
 
 
@@ -542,12 +560,12 @@ def main() -> int:
     rel_err64 = float(np.max(np.abs(tvs_b64 - tvs_p) / tvs_b64))
     bit_exact = bool(np.array_equal(tvs_b.view(np.uint32), tvs_p.view(np.uint32)))
 
-    bytes_per_launch = real_point_steps * 2     # one int16 height per executed step
-    achieved = bytes_per_launch / (b_kernel_ms / 1000.0) / 1e9
+    gather_bytes = real_point_steps * 2     # one int16 height per executed step, L2-served
     per_step = VALU_PER_STEP["short"] if short_sectors == 180 else VALU_PER_STEP["split"]
     if shape["two_points"]:
         per_step += 1   # the two heights of a dword are converted by one instruction each
-    valu_lane_ops = real_point_steps * per_step / (b_kernel_ms / 1000.0)
+    lane_ops = real_point_steps * per_step          # VALU lane-operations per launch
+    valu_lane_ops = lane_ops / (b_kernel_ms / 1000.0)
     traffic = None
     prof = os.path.join(ROOT, "profiles", "walk_tiles_main-1201_traffic.json")
     if os.path.exists(prof):
@@ -585,25 +603,32 @@ def main() -> int:
                                                  "pipeline", "persistent", "both_directions",
                                                  "two_points")},
         },
+        # The walk is bound by vector-instruction issue, not HBM: its height
+        # gathers hit L2 (99.4 % of them), and the HBM traffic it makes is
+        # ~1 % of the gathered bytes.
         "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBPS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBPS,
+            "bound": "valu",
+            "achieved": valu_lane_ops / 1e12,
+            "peak": VALU_ISSUE_PEAK_LANE_OPS / 1e12,
+            "unit": "T lane-ops/s (wave64 VALU instructions x 64)",
+            "frac": valu_lane_ops / VALU_ISSUE_PEAK_LANE_OPS,
             "traffic": traffic,
             "kernel": "walk_tiles_kernel<int16_t> (batched walk)",
             "kernel_ms": b_kernel_ms,
-            "bytes_per_launch": bytes_per_launch,
+            "valu_per_band_step": per_step,
+            "short_divide_sectors": short_sectors,
+            "lane_ops_per_launch": lane_ops,
+            "peak_basis": "1024 SIMDs x 2 wave64 VALU instructions per 4 clocks (dual issue) "
+                          "x 2.4 GHz x 64 lanes",
+            "hbm": {"bytes_per_launch": traffic,
+                    "achieved_GBps": (traffic / (b_kernel_ms / 1000.0) / 1e9
+                                      if traffic else None),
+                    "peak_GBps": HBM_PEAK_GBPS,
+                    "frac": (traffic / (b_kernel_ms / 1000.0) / 1e9 / HBM_PEAK_GBPS
+                             if traffic else None)},
+            "l2_served_gather_bytes_per_launch": gather_bytes,
         },
         "nominal_value": point_steps / (b_ms / 1000.0),
-        "valu": {
-            "instructions_per_step": per_step,
-            "short_divide_sectors": short_sectors,
-            "achieved_lane_ops_per_s": valu_lane_ops,
-            "peak_lane_ops_per_s": VALU_PEAK_LANE_OPS,
-            "frac": valu_lane_ops / VALU_PEAK_LANE_OPS,
-        },
         "plain": plain,
         "batched_f32": batched_f32,
         "tvs_rel_err_batched_vs_plain": rel_err,
@@ -626,8 +651,12 @@ def main() -> int:
         # single-thread sample.
         cores = min(16, len(os.sched_getaffinity(0)))
         if cores > 1:
-            out["cpu_all_cores"] = cpu_baseline(wl, 180, cores)
-            log(f"CPU, {cores} threads: {out['cpu_all_cores']['value']:.3e} point-steps/s")
+            out["cpu_all_cores"], cpu_surf = cpu_baseline(wl, 180, cores, keep_surfaces=True)
+            # The oracle's 180-sector TVS against the batched pass checked above.
+            out["tvs_bit_exact_vs_oracle"] = bool(
+                np.array_equal(cpu_surf.view(np.uint32), tvs_b.view(np.uint32)))
+            log(f"CPU, {cores} threads: {out['cpu_all_cores']['value']:.3e} point-steps/s; "
+                f"batched TVS bit-exact vs the oracle: {out['tvs_bit_exact_vs_oracle']}")
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
