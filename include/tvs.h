@@ -211,6 +211,20 @@ tvs_status tvs_compute_axes(tvs_ctx *ctx, uint32_t sector, float *distances,
 tvs_status tvs_run_sectors(tvs_ctx *ctx, uint32_t first_sector,
                            uint32_t sector_count, tvs_method method);
 
+/* tvs_run_sectors with TVS_METHOD_BATCHED, allocating nothing on the
+ * device.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (serving loops, code that must not call hipMalloc).
+ * The per-sector band surfaces go to the library's static 64 MiB device
+ * scratch instead of a buffer grown on demand, sectors walked in chunks that
+ * fit it; results are bit-identical to tvs_run_sectors.  Limits: one
+ * sector's band surfaces (2N x 4 bytes) must fit the scratch (N <= 8388608,
+ * else TVS_ERR_SIZE); no ring capture (TVS_ERR_INVALID_ARGUMENT when
+ * ring_capture_sectors > 0); a directional walk shape runs in the
+ * XCD-contiguous order.  Synchronous: returns once the run has finished.
+ * Calls on one device share its scratch and are serialised by a lock. */
+tvs_status tvs_run_sectors_noalloc(tvs_ctx *ctx, uint32_t first_sector,
+                                   uint32_t sector_count);
+
 /* One sector from host band data, returning the cumulative surfaces:
  * exactly `GPU::run` (gpu.rs:387-468) as called by compute_sector_vulkan
  * (compute.rs:217-225).  `cumulative_out` may be NULL.  Synchronous. */

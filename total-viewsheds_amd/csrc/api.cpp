@@ -288,9 +288,12 @@ tvs_status ensure_short_div(tvs_ctx *c, const StepTables &tab, uint32_t first, u
 }
 
 // Descriptor of this context's DEM for a batched launch over sector slots
-// [first, first + count); grows the band-surface buffer as needed.
+// [first, first + count); grows the band-surface buffer as needed.  With
+// `allocate` false nothing is allocated: the caller sets `partial` itself
+// and the shape must not be directional (its block order is device memory
+// built on demand).
 tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t capture,
-                        const WalkConfig &cfg, TileDesc *out) {
+                        const WalkConfig &cfg, TileDesc *out, bool allocate) {
     const uint32_t N = c->geo.computable_points_count;
     if (!walk_shape_supported(cfg))
         return fail(c, TVS_ERR_INVALID_ARGUMENT, "walk shape not compiled");
@@ -303,7 +306,9 @@ tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t cap
         return fail(c, TVS_ERR_SIZE,
                     "DEM too wide for the batched walk's 32-bit height offsets (int16 up to "
                     "32768 points wide, f32 up to 23170); use TVS_METHOD_PLAIN");
-    if (c->partial_cap < count) {
+    if (!allocate && cfg.directional)
+        return fail(c, TVS_ERR_INVALID_ARGUMENT, "directional block order needs allocation");
+    if (allocate && c->partial_cap < count) {
         dev_free(c->partial);
         c->partial_cap = 0;
         TVS_HIP(c, dev_alloc(&c->partial, static_cast<size_t>(count) * 2ull * N),
@@ -348,7 +353,7 @@ tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t cap
     d.ring_offsets = nullptr;
     d.ring_events = nullptr;
     d.order = nullptr;
-    if (cfg.directional && cfg.xcd_contiguous && !cfg.persistent) {
+    if (allocate && cfg.directional && cfg.xcd_contiguous && !cfg.persistent) {
         const uint64_t key = (1ull << 63) | (uint64_t(cfg.twin) << 48) |
                              (uint64_t(cfg.block) << 32) | (uint64_t(d.w) << 16) | d.T;
         if (c->order_key != key) {

@@ -16,12 +16,6 @@
 
 #include "engine.hpp"
 
-namespace tvs {
-tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t capture,
-                        const WalkConfig &cfg, TileDesc *out);
-tvs_status ensure_short_div(tvs_ctx *c, const StepTables &tab, uint32_t first, uint32_t count);
-}
-
 struct tvs_batch {
     std::vector<tvs_ctx *> tiles;
     std::vector<uint32_t> owner;     // tile whose band tables a tile uses

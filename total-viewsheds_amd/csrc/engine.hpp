@@ -330,7 +330,10 @@ tvs_status run_sectors(tvs_ctx *c, uint32_t first, uint32_t count, tvs_method me
                        bool keep_rings);
 tvs_status validate_slots(tvs_ctx *c, uint32_t first, uint32_t count);
 tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t capture,
-                        const WalkConfig &cfg, TileDesc *out);
+                        const WalkConfig &cfg, TileDesc *out, bool allocate = true);
+// Verifies the short divide for the unchecked slots of [first, first +
+// count) against step tables `tab` (launches only; allocates nothing).
+tvs_status ensure_short_div(tvs_ctx *c, const StepTables &tab, uint32_t first, uint32_t count);
 
 // ---- host geometry (geometry.cpp) ----
 tvs_status dem_geometry(uint32_t width, float scale, uint32_t los,

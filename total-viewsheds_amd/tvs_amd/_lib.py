@@ -149,6 +149,7 @@ def load() -> ctypes.CDLL:
         "tvs_get_band": (st, [vp, ctypes.c_uint32, vp, vp, ctypes.c_size_t]),
         "tvs_compute_axes": (st, [vp, ctypes.c_uint32, vp, vp, vp]),
         "tvs_run_sectors": (st, [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]),
+        "tvs_run_sectors_noalloc": (st, [vp, ctypes.c_uint32, ctypes.c_uint32]),
         "tvs_run_sector_with_band": (st, [vp, vp, vp, ctypes.c_size_t, vp, ctypes.c_size_t]),
         "tvs_reset_surfaces": (st, [vp]),
         "tvs_read_surfaces": (st, [vp, vp, ctypes.c_size_t]),
@@ -343,6 +344,11 @@ class Engine:
     # -- walk --
     def run(self, first: int = 0, count: int = 180, method: str = "batched") -> None:
         self._c(load().tvs_run_sectors(self._h, first, count, METHODS[method]))
+
+    def run_noalloc(self, first: int = 0, count: int = 180) -> None:
+        """tvs_run_sectors_noalloc: the batched run with no device allocation
+        (band surfaces in the library's static scratch); synchronous."""
+        self._c(load().tvs_run_sectors_noalloc(self._h, first, count))
 
     def run_sector_with_band(self, deltas: np.ndarray, dists: np.ndarray) -> np.ndarray:
         d = np.ascontiguousarray(deltas, np.int32)
