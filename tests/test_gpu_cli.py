@@ -97,3 +97,67 @@ def test_band_cache_round_trip(tmp_path, small):
         with pytest.raises(tvs_amd.TvsError) as err:
             e.cache_load(str(tmp_path), 5)
         assert err.value.status == _lib.ERR_SIZE
+
+
+@pytest.fixture(scope="module")
+def running_total(oracle, small):
+    """compute.rs:206-214 on the oracle: the f32 running total of the
+    cumulative surfaces after every sector (what the per-sector heatmap
+    shows)."""
+    wl, dem, _ = small
+    elev = wl.elevations.astype(np.float32)
+    cum = np.zeros(dem.computable_points_count, np.float32)
+    total = np.zeros_like(cum)
+    for s in range(180):
+        oracle.compute_run(dem, elev, wl.observer_height, wl.rings_per_km, first_sector=s,
+                           sector_count=1, rings=False, n_threads=8, cumulative=cum)
+        total += cum
+    return total
+
+
+def test_compute_mirror_cache_and_heatmap(tmp_path, oracle, small, running_total):
+    """The Python mirror of `Compute` (compute.rs:101-178): with a state
+    directory the band data of every sector is computed once and saved in
+    the reference's cache layout, then loaded by the next run; with an output
+    directory heatmap.png shows the running total (compute.rs:206-214)."""
+    from tvs_amd.compute import DEM, Compute
+    wl, dem, cum = small
+    d = DEM(wl.width, wl.spacing, wl.max_line_of_sight)
+    d.elevations = wl.elevations
+    state, out = tmp_path / "state", tmp_path / "out"
+    c = Compute("hip", state, out, d, wl.rings_per_km, wl.observer_height, capture_rings=False)
+    surfaces, _ = c.run()
+    assert c.sectors_loaded == 0
+    np.testing.assert_array_equal(surfaces, cum)
+    np.testing.assert_array_equal(c.total_surfaces, running_total)
+    np.testing.assert_array_equal(read_png_grey(str(out / "heatmap.png")),
+                                  png_of(running_total, dem.tvs_width))
+    cache = state / "total-viewsheds" / "201"
+    deltas, dists = oracle.band_data(dem, 37)
+    assert (cache / "band_deltas" / "37.bin").read_bytes() == deltas.tobytes()
+    assert (cache / "distances" / "37.bin").read_bytes() == dists.tobytes()
+    # The next run loads all 180 sectors; a file of another band length is
+    # recomputed and rewritten.
+    f = cache / "distances" / "5.bin"
+    f.write_bytes(f.read_bytes()[:-4])
+    c2 = Compute("hip-plain", state, None, d, wl.rings_per_km, wl.observer_height,
+                 capture_rings=False)
+    surfaces2, _ = c2.run()
+    assert c2.sectors_loaded == 179
+    np.testing.assert_array_equal(surfaces2, cum)
+    assert f.read_bytes() == oracle.band_data(dem, 5)[1].tobytes()
+
+
+def test_cli_heatmap_each_sector(tmp_path, small, running_total):
+    """--heatmap-each-sector redraws heatmap.png after every sector from the
+    running total, as compute.rs:106-112 does."""
+    wl, dem, _ = small
+    path = tmp_path / "dem.bt"
+    bt.write(str(path), wl.elevations.reshape(wl.width, wl.width))
+    out = tmp_path / "out"
+    r = subprocess.run([CLI, "--input", str(path), "--scale", "100", "--max-line-of-sight",
+                        "6700", "--output-dir", str(out), "--no-cache",
+                        "--heatmap-each-sector"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    np.testing.assert_array_equal(read_png_grey(str(out / "heatmap.png")),
+                                  png_of(running_total, dem.tvs_width))
