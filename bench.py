@@ -135,7 +135,7 @@ def measure_large(steps: int, warmup: int, cpu: bool = False) -> dict:
         e.upload(wl.elevations)
         e.compute_bands(0, 180)
         bands_ms = e.timing().bands_ms
-        shape, _ = e.autotune(0, 180, reps=1)
+        shape, _ = e.autotune(0, 180, reps=3)
         ms, kernel_ms, tm = time_method(e, "batched", steps, warmup)
         e.compact_rings(45)                       # allocates; another sector
         t0 = time.perf_counter()
@@ -200,7 +200,7 @@ def measure_tiles(steps: int, warmup: int, cpu: bool = False) -> dict:
             b.upload(i, t.elevations)
         b.compute_bands()
         bands_ms = b.timing().bands_ms
-        shape, _ = b.autotune(reps=1)
+        shape, _ = b.autotune(reps=3)
         log(f"tiles-64 tuned shape: block {shape['block']}, unroll {shape['unroll']}, "
             f"xcd {shape['xcd_contiguous']}, directional {shape['directional']}, both directions {shape['both_directions']}, "
             f"two points {shape['two_points']}")
@@ -431,8 +431,10 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true", help="keep the default walk shape")
     ap.add_argument("--shape", metavar="B,U,X,P,S,D,T",
                     help="walk shape instead of tuning: points per item, unroll, "
-                         "XCD-contiguous, pipeline, persistent, both directions per lane, "
-                         "two points per lane (e.g. 64,8,1,0,0,0,0)")
+                         "item order (0 round-robin, 1 XCD-contiguous, 2 directional), "
+                         "pipeline, persistent, both directions per lane, two points per "
+                         "lane (0, 1, or 2 = two pairs from adjacent rows) "
+                         "(e.g. 64,8,1,0,0,1,1)")
     ap.add_argument("--no-f32", action="store_true", help="skip the f32-heights batched run")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the large-3601 and tiles-64 figures")
@@ -484,10 +486,10 @@ def main() -> int:
 
     if args.shape:
         b, u, x, p, ps, dd, tw = (int(v) for v in args.shape.split(","))
-        e.set_walk_shape(b, u, bool(x), bool(p), bool(ps), bool(dd), bool(tw))
+        e.set_walk_shape(b, u, x, p, ps, dd, tw)
     shape = e.walk_shape()
     if not args.no_tune and not args.shape:
-        shape, table = e.autotune(0, 180, reps=1)
+        shape, table = e.autotune(0, 180, reps=3)
         log(f"autotuned walk shape: block {shape['block']}, unroll {shape['unroll']}, "
             f"xcd-contiguous {shape['xcd_contiguous']}, directional {shape['directional']}, pipeline {shape['pipeline']}, "
             f"persistent {shape['persistent']}, both directions {shape['both_directions']}, "
@@ -527,7 +529,7 @@ def main() -> int:
                             wl.observer_height, 0) as ef:
             ef.upload(wl.elevations.astype(np.float32) + np.float32(0.25))
             ef.compute_bands(0, 180)
-            f_shape, _ = ef.autotune(0, 180, reps=1)   # f32 heights: their own best shape
+            f_shape, _ = ef.autotune(0, 180, reps=3)   # f32 heights: their own best shape
             f_ms, f_kernel_ms, _ = time_method(ef, "batched", args.steps, args.warmup)
             ef.reset_surfaces()
             ef.run(0, 180, "batched")

@@ -325,10 +325,11 @@ tvs_status tvs_get_walk_shape(const tvs_ctx *ctx, tvs_walk_shape *out);
 size_t tvs_walk_shape_count(void);
 
 /* Time every compiled shape on the context's own DEM over sectors
- * [first, first+count) (`reps` timed runs after one warm-up each), keep the
- * fastest as the context's shape, return all timings in `results` (up to
- * `max_results`) and the winner in `best`.  Cumulative surfaces are
- * preserved. */
+ * [first, first+count) (one warm-up run, then the fastest of `reps` timed
+ * runs), keep the fastest shape as the context's, return all timings in
+ * `results` (up to `max_results`) and the winner in `best`.  Cumulative
+ * surfaces, ring records and the ring overflow count are preserved (the
+ * timing runs record no rings). */
 tvs_status tvs_autotune(tvs_ctx *ctx, uint32_t first_sector, uint32_t sector_count,
                         uint32_t reps, tvs_walk_shape *results, size_t max_results,
                         size_t *n_results, tvs_walk_shape *best);

@@ -71,7 +71,7 @@ def check_loop(loop: str, load: str, band_steps: int) -> None:
 def test_walk_loop_shape(asm, heights, load, pipe):
     """One band per lane, 8 steps per chunk: the split-reciprocal loop spends
     9 VALU per step, the short-divide loop 7, the IEEE loop more."""
-    loops = unrolled_loops(kernel_body(asm, heights + "Lb0ELi256ELi8E" + pipe + "Lb0ELb0ELb0ELb0E"))
+    loops = unrolled_loops(kernel_body(asm, heights + "Lb0ELi256ELi8E" + pipe + "Lb0ELb0ELi0ELb0E"))
     fast = [l for l in loops if "v_div_scale" not in l]
     assert len(fast) == 2
     for loop in fast:
@@ -85,7 +85,7 @@ def test_paired_walk_loop_shape(asm):
     """Both bands of a point per lane: 16 height loads and 16 masked updates
     per 8-step chunk, one set of table reads (pos_f, pos_b + the shared
     constants), 9 (split reciprocal) or 7 (short divide) VALU per band-step."""
-    loops = unrolled_loops(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb1ELb0ELb0E"))
+    loops = unrolled_loops(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb1ELi0ELb0E"))
     fast = [l for l in loops if "v_div_scale" not in l]
     assert len(fast) == 2
     for loop in fast:
@@ -100,7 +100,7 @@ def test_two_point_walk_loop_shape(asm):
     """Two adjacent points per lane: one dword load per step fetches both
     int16 heights (8 loads for 16 band-steps), each converted by one SDWA
     instruction; 8 (short divide) or 10 VALU per band-step."""
-    loops = unrolled_loops(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb0ELb1ELb0E"))
+    loops = unrolled_loops(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb0ELi1ELb0E"))
     twin = [l for l in loops if "v_div_scale" not in l and "buffer_load_dword" in l]
     assert len(twin) == 2
     for loop in twin:
@@ -114,6 +114,44 @@ def test_two_point_walk_loop_shape(asm):
     assert valu(short) <= 8 * 16 + 8 and valu(split) <= 10 * 16 + 8, (valu(short), valu(split))
 
 
+def scalar_operand_valu(loop: str) -> int:
+    """Vector instructions of a loop that read a scalar register."""
+    return sum(1 for args in re.findall(r"^\s+v_\w+\s+([^\n]*)", loop, re.M)
+               if re.search(r"\bs\d+\b|\bs\[\d", args))
+
+
+def test_eight_band_walk_loop_shape(asm):
+    """Two pairs of adjacent points per lane (rows 2r and 2r + 1), both
+    directions, 4 steps per chunk: 16 dword loads (one per pair, direction
+    and step) and 32 masked updates per chunk; the step's constants are
+    copied into vector registers once per step, so only those copies (4 per
+    step with the short divide, 5 with the split reciprocal) read a scalar
+    register, against half the instructions of the four-band loop; 8.5
+    vector instructions per band-step with the short divide."""
+    loops = unrolled_loops(kernel_body(asm, "IsLb0ELi64ELi4ELb0ELb0ELb1ELi2ELb0E"))
+    # (the kernel also holds the one-point loops of odd widths' left-over
+    # column, which load with buffer_load_format_x)
+    fast = [l for l in loops if "v_div_scale" not in l and "buffer_load_format_x" not in l]
+    assert len(fast) == 2
+    for loop in fast:
+        assert "s_cbranch_execz" not in loop and "s_cbranch_vccz" not in loop
+        assert len(re.findall(r"buffer_load_dword\s+v\d+, v\d+, s\[\d+:\d+\], s\d+ offen",
+                              loop)) == 16
+        assert len(re.findall(r"v_cmpx_gt_f32", loop)) == 32
+        assert "scratch" not in loop and "buffer_store" not in loop
+    short, split = sorted(fast, key=valu)
+    assert valu(short) <= 272 + 8 and valu(split) <= 336 + 8, (valu(short), valu(split))
+    assert scalar_operand_valu(short) <= 16 and scalar_operand_valu(split) <= 20
+    four = unrolled_loops(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb1ELi1ELb0E"))
+    four_short = min((l for l in four if "v_div_scale" not in l), key=valu)
+    assert scalar_operand_valu(four_short) >= valu(four_short) // 2 - 8
+
+
+def test_eight_band_kernel_holds_five_waves(asm):
+    info = kernel_info(asm, "IsLb0ELi64ELi4ELb0ELb0ELb1ELi2ELb0E")
+    assert info["Occupancy"] >= 5 and info["ScratchSize"] == 0, info
+
+
 def kernel_info(asm: str, mangled_part: str) -> dict:
     """The compiler's register and occupancy summary of one kernel."""
     m = re.search(r"^(_Z\S*walk_tiles_kernel" + mangled_part + r"\S*):", asm, re.M)
@@ -123,9 +161,9 @@ def kernel_info(asm: str, mangled_part: str) -> dict:
     return {k: int(v) for k, v in re.findall(r"; (\w+): (\d+)", block)}
 
 
-@pytest.mark.parametrize("shape", ["Li64ELi4ELb0ELb0ELb1ELb1E", "Li64ELi8ELb0ELb0ELb1ELb1E",
-                                   "Li128ELi4ELb0ELb0ELb1ELb1E", "Li64ELi8ELb0ELb0ELb0ELb1E",
-                                   "Li256ELi8ELb0ELb0ELb0ELb0E"])
+@pytest.mark.parametrize("shape", ["Li64ELi4ELb0ELb0ELb1ELi1E", "Li64ELi8ELb0ELb0ELb1ELi1E",
+                                   "Li128ELi4ELb0ELb0ELb1ELi1E", "Li64ELi8ELb0ELb0ELb0ELi1E",
+                                   "Li256ELi8ELb0ELb0ELb0ELi0E"])
 def test_fast_divide_kernels_keep_seven_waves(asm, shape):
     """The fast-divide kernels walk a tile without the fast divides one IEEE
     step at a time, so their registers are those of the fast loops: at most
@@ -140,9 +178,9 @@ def test_ieee_only_kernel_unrolls_the_ieee_loop(asm):
     """The IEEE-only instantiation holds the unrolled IEEE loop (16 loads of
     two heights, IEEE divide sequences) and no fast-divide loop; the
     fast-divide instantiation holds both fast loops and no unrolled IEEE loop."""
-    ieee = unrolled_loops(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb0ELb1ELb1E"))
+    ieee = unrolled_loops(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb0ELi1ELb1E"))
     assert ieee and all("v_div_scale" in l for l in ieee)
-    fast = unrolled_loops(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb0ELb1ELb0E"))
+    fast = unrolled_loops(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb0ELi1ELb0E"))
     assert not any("v_div_scale" in l for l in fast)
     assert len([l for l in fast if "buffer_load_dword" in l]) == 2
 

@@ -155,8 +155,10 @@ def test_fractional_heights_fast_divide(oracle, case):
 def test_two_points_per_lane_even_and_odd_widths(oracle, w, los):
     """Two points per lane pairs columns (2c, 2c + 1), shifted one column on
     odd rows of an odd-width DEM (whose left-over column then alternates
-    between the last and the first): even and odd widths, each with an even
-    and an odd L (100 m grid: L = 32 or 33), give the oracle's surfaces."""
+    between the last and the first); four points per lane take such pairs
+    from rows 2r and 2r + 1 (the last row alone when T is odd).  Even and
+    odd widths, each with an even and an odd L (100 m grid: L = 32 or 33),
+    give the oracle's surfaces."""
     rng = np.random.default_rng(w)
     elev = rng.integers(0, 900, size=w * w).astype(np.int16)
     note(f"two points per lane {w}x{w}, L={los // 100}: oracle band data + walk")
@@ -167,10 +169,13 @@ def test_two_points_per_lane_even_and_odd_widths(oracle, w, los):
     with tvs_amd.Engine(w, 100.0, los, 5.0, 1.65, 0) as e:
         e.upload(elev)
         e.compute_bands(0, 180)
-        for block, unroll, pers, pair, xcd in ((64, 4, False, True, 1), (128, 8, False, False, 1),
-                                               (64, 8, True, False, 1), (64, 8, False, True, 2)):
-            note(f"  shape {block}x{unroll} persistent={pers} pair={pair} xcd={xcd} two points")
-            e.set_walk_shape(block, unroll, xcd, False, pers, pair, True)
+        for block, unroll, pers, pair, xcd, two in (
+                (64, 4, False, True, 1, 1), (128, 8, False, False, 1, 1), (64, 8, True, False, 1, 1),
+                (64, 8, False, True, 2, 1), (64, 4, False, True, 1, 2), (64, 4, False, True, 2, 2),
+                (64, 4, False, True, 0, 2)):
+            note(f"  shape {block}x{unroll} persistent={pers} pair={pair} xcd={xcd} "
+                 f"points per lane {2 * two}")
+            e.set_walk_shape(block, unroll, xcd, False, pers, pair, two)
             e.reset_surfaces()
             e.run(0, 180, "batched")
             np.testing.assert_array_equal(e.surfaces().view(np.uint32), cum.view(np.uint32),
@@ -529,7 +534,7 @@ SHAPES = [  # (block, unroll, pipeline, persistent, both directions, two points 
     (128, 8, False, False, False, True), (64, 4, False, False, False, True),
     (64, 4, False, False, True, True), (128, 4, False, False, True, True),
     (64, 8, False, False, True, True), (64, 4, False, True, True, True),
-    (64, 8, False, True, False, True)]
+    (64, 8, False, True, False, True), (64, 4, False, False, True, 2)]
 
 
 # xcd: 0 round-robin items, 1 XCD-contiguous, 2 XCD-contiguous in the

@@ -280,7 +280,7 @@ tvs_status tvs_batch_set_walk_shape(tvs_batch *b, const tvs_walk_shape *shape) {
         return bfail(b, TVS_ERR_INVALID_ARGUMENT, "xcd_contiguous must be 0, 1 or 2");
     tvs::WalkConfig cfg{shape->block, shape->unroll, shape->xcd_contiguous != 0,
                         shape->pipeline != 0, shape->persistent != 0,
-                        shape->both_directions != 0, shape->two_points != 0,
+                        shape->both_directions != 0, shape->two_points,
                         shape->xcd_contiguous == 2u};
     if (!tvs::walk_shape_supported(cfg))
         return bfail(b, TVS_ERR_INVALID_ARGUMENT, "walk shape not compiled");
@@ -318,21 +318,25 @@ tvs_status tvs_batch_autotune(tvs_batch *b, uint32_t reps, tvs_walk_shape *resul
                                           b->i16, false, b->ieee_only, b->cfg);
         };
         hipError_t e = launch();   // warm-up
-        if (e == hipSuccess) e = hipEventRecord(b->ev[0], b->stream);
-        for (uint32_t r = 0; r < reps && e == hipSuccess; ++r) e = launch();
-        if (e == hipSuccess) e = hipEventRecord(b->ev[1], b->stream);
-        if (e == hipSuccess) e = hipEventSynchronize(b->ev[1]);
-        float ms = 0.0f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
+        // The fastest of the timed launches (as tvs_autotune).
+        float ms = 1e30f;
+        for (uint32_t r = 0; r < reps && e == hipSuccess; ++r) {
+            e = hipEventRecord(b->ev[0], b->stream);
+            if (e == hipSuccess) e = launch();
+            if (e == hipSuccess) e = hipEventRecord(b->ev[1], b->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(b->ev[1]);
+            float one = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&one, b->ev[0], b->ev[1]);
+            if (e == hipSuccess) ms = std::min(ms, one);
+        }
         if (e != hipSuccess) {
             st = bfail(b, TVS_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
             break;
         }
-        ms /= static_cast<float>(reps);
         if (results && k < max_results)
             results[k] = tvs_walk_shape{cfg.block, cfg.unroll, tvs::xcd_mode(cfg),
                                         cfg.pipeline ? 1u : 0u, cfg.persistent ? 1u : 0u,
-                                        cfg.pair ? 1u : 0u, cfg.twin ? 1u : 0u, ms};
+                                        cfg.pair ? 1u : 0u, cfg.twin, ms};
         ++k;
         if (ms < winner_ms) {
             winner_ms = ms;
@@ -348,7 +352,7 @@ tvs_status tvs_batch_autotune(tvs_batch *b, uint32_t reps, tvs_walk_shape *resul
     if (best)
         *best = tvs_walk_shape{winner.block, winner.unroll, tvs::xcd_mode(winner),
                                winner.pipeline ? 1u : 0u, winner.persistent ? 1u : 0u,
-                               winner.pair ? 1u : 0u, winner.twin ? 1u : 0u, winner_ms};
+                               winner.pair ? 1u : 0u, winner.twin, winner_ms};
     return TVS_OK;
 }
 

@@ -132,7 +132,9 @@ struct WalkConfig {
     bool pipeline = false;    // next chunk's heights requested before this chunk's math
     bool persistent = false;  // a few blocks per CU stride over all items
     bool pair = false;        // one lane walks both bands (forward, backward) of a point
-    bool twin = false;        // one lane walks two adjacent points of a row
+    // Points per lane: 0 one; 1 two adjacent points of a row; 2 two such
+    // pairs from two adjacent rows (with `pair`: eight bands per lane).
+    uint32_t twin = 0;
     // With xcd_contiguous: the work items of each sector taken in the order
     // of their position across the sector's band direction, so that the
     // items in flight on one XCD form a strip along that direction and
@@ -166,14 +168,21 @@ inline WalkConfig capture_walk_config(const WalkConfig &cfg) {
 // the tile walk for N = T*T points: BLOCK points per block, or with `twin`
 // the T*(T/2) column pairs two per lane plus (odd T) the T last-column
 // points one per lane.
+// Lane groups of the two-point walks: the column pairs of every row (twin
+// 1), or of every two adjacent rows (twin 2: rows 2r and 2r + 1, the last
+// row alone when T is odd).
+__host__ __device__ inline uint32_t twin_lane_groups(uint32_t T, uint32_t twin) {
+    return twin == 2u ? ((T + 1u) / 2u) * (T / 2u) : T * (T / 2u);
+}
+
 inline uint32_t walk_blocks_per_pass(uint32_t T, const WalkConfig &cfg) {
     if (!cfg.twin) return (T * T + cfg.block - 1) / cfg.block;
-    const uint32_t pairs = T * (T / 2u);
-    return (pairs + cfg.block - 1) / cfg.block + ((T & 1u) ? (T + cfg.block - 1) / cfg.block : 0u);
+    const uint32_t groups = twin_lane_groups(T, cfg.twin);
+    return (groups + cfg.block - 1) / cfg.block + ((T & 1u) ? (T + cfg.block - 1) / cfg.block : 0u);
 }
 
 // Compiled (block, unroll, pipeline, persistent, pair, twin) shapes of the
-// tile walk.
+// tile walk (twin: points per lane as WalkConfig::twin).
 #define TVS_WALK_SHAPES(X)                                                               \
     X(256, 8, false, false, false, false) X(128, 8, false, false, false, false)          \
     X(64, 8, false, false, false, false) X(512, 8, false, false, false, false)           \
@@ -188,7 +197,7 @@ inline uint32_t walk_blocks_per_pass(uint32_t T, const WalkConfig &cfg) {
     X(128, 8, false, false, false, true) X(64, 4, false, false, false, true)             \
     X(64, 4, false, false, true, true) X(128, 4, false, false, true, true)               \
     X(64, 8, false, false, true, true) X(64, 4, false, true, true, true)                 \
-    X(64, 8, false, true, false, true)
+    X(64, 8, false, true, false, true) X(64, 4, false, false, true, 2)
 
 // Scratch for the device Axes::compute (keys, ids, radix-sort ping-pong,
 // histograms).

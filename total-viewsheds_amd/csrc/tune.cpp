@@ -22,9 +22,9 @@ std::vector<WalkConfig> compiled_walk_shapes() {
     // A persistent grid strides over the items itself: one item mapping.
     // The directional order is timed for the two-points-per-lane shapes
     // (the int16 walk's fastest family).
-#define TVS_SHAPE(B, U, P, S, D, W)                                        \
-    if (!S) out.push_back(WalkConfig{B, U, true, P, S, D, W});        \
-    if (!S && W) out.push_back(WalkConfig{B, U, true, P, S, D, W, true}); \
+#define TVS_SHAPE(B, U, P, S, D, W)                                                  \
+    if (!S) out.push_back(WalkConfig{B, U, true, P, S, D, W});                  \
+    if (!S && W) out.push_back(WalkConfig{B, U, true, P, S, D, W, true});       \
     out.push_back(WalkConfig{B, U, false, P, S, D, W});
     TVS_WALK_SHAPES(TVS_SHAPE)
 #undef TVS_SHAPE
@@ -47,10 +47,17 @@ std::vector<uint32_t> directional_walk_order(uint32_t T, uint32_t w, const WalkC
         col[k] = 0.5 * (c0 + c1);
     };
     if (cfg.twin) {
-        const uint64_t half = T / 2u, n_pairs = uint64_t(T) * half;
-        const uint32_t twin_blocks = static_cast<uint32_t>((n_pairs + B - 1) / B);
+        const uint64_t half = T / 2u, n_groups = twin_lane_groups(T, cfg.twin);
+        const uint32_t twin_blocks = static_cast<uint32_t>((n_groups + B - 1) / B);
+        // Centre of a lane group: one column pair, or (twin 2) the column
+        // pairs of rows 2r and 2r + 1.
         auto pair_rc = [&](uint64_t p, double &r, double &cc) {
             const uint64_t rr = p / half;
+            if (cfg.twin == 2u) {
+                r = 2.0 * static_cast<double>(rr) + 0.5;
+                cc = 2.0 * static_cast<double>(p - rr * half) + 1.0;
+                return;
+            }
             r = static_cast<double>(rr);
             cc = 2.0 * static_cast<double>(p - rr * half) + ((w & rr & 1u) ? 1.5 : 0.5);
         };
@@ -59,7 +66,7 @@ std::vector<uint32_t> directional_walk_order(uint32_t T, uint32_t w, const WalkC
             if (k < twin_blocks) {
                 const uint64_t p0 = uint64_t(k) * B;
                 pair_rc(p0, r0, c0);
-                pair_rc(std::min<uint64_t>(p0 + B, n_pairs) - 1u, r1, c1);
+                pair_rc(std::min<uint64_t>(p0 + B, n_groups) - 1u, r1, c1);
             } else {   // each row's left-over column (odd T), one point per lane
                 const uint64_t i0 = uint64_t(k - twin_blocks) * B;
                 r0 = static_cast<double>(i0);
@@ -108,7 +115,7 @@ tvs_status tvs_set_walk_shape(tvs_ctx *c, const tvs_walk_shape *shape) {
         return tfail(c, TVS_ERR_INVALID_ARGUMENT, "xcd_contiguous must be 0, 1 or 2");
     tvs::WalkConfig cfg{shape->block, shape->unroll, shape->xcd_contiguous != 0,
                         shape->pipeline != 0, shape->persistent != 0,
-                        shape->both_directions != 0, shape->two_points != 0,
+                        shape->both_directions != 0, shape->two_points,
                         shape->xcd_contiguous == 2u};
     if (!tvs::walk_shape_supported(cfg))
         return tfail(c, TVS_ERR_INVALID_ARGUMENT, "walk shape not compiled");
@@ -126,7 +133,7 @@ tvs_status tvs_get_walk_shape(const tvs_ctx *c, tvs_walk_shape *out) {
     out->pipeline = cfg.pipeline ? 1u : 0u;
     out->persistent = cfg.persistent ? 1u : 0u;
     out->both_directions = cfg.pair ? 1u : 0u;
-    out->two_points = cfg.twin ? 1u : 0u;
+    out->two_points = cfg.twin;
     out->ms = 0.0f;
     return TVS_OK;
 }
@@ -167,19 +174,20 @@ tvs_status tvs_autotune(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t rep
         // time the one capture kernel) and leave the caller's rings as they
         // are.
         st = tvs::run_sectors(c, first, count, TVS_METHOD_BATCHED, true);   // warm-up
-        float total = 0.0f;
+        // The fastest of the timed runs: a run slowed by something else on
+        // the device (clocks still rising, another context) does not decide.
+        float ms = 1e30f;
         for (uint32_t r = 0; r < reps && st == TVS_OK; ++r) {
             st = tvs::run_sectors(c, first, count, TVS_METHOD_BATCHED, true);
             tvs_timing t;
             if (st == TVS_OK) st = tvs_get_timing(c, &t);
-            total += t.walk_ms;
+            if (st == TVS_OK) ms = std::min(ms, t.walk_ms);
         }
         if (st != TVS_OK) break;
-        const float ms = total / static_cast<float>(reps);
         if (results && k < max_results)
             results[k] = tvs_walk_shape{cfg.block, cfg.unroll, tvs::xcd_mode(cfg),
                                         cfg.pipeline ? 1u : 0u, cfg.persistent ? 1u : 0u,
-                                        cfg.pair ? 1u : 0u, cfg.twin ? 1u : 0u, ms};
+                                        cfg.pair ? 1u : 0u, cfg.twin, ms};
         ++k;
         if (ms < winner_ms) {
             winner_ms = ms;
@@ -200,7 +208,7 @@ tvs_status tvs_autotune(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t rep
     if (best)
         *best = tvs_walk_shape{winner.block, winner.unroll, tvs::xcd_mode(winner),
                                winner.pipeline ? 1u : 0u, winner.persistent ? 1u : 0u,
-                               winner.pair ? 1u : 0u, winner.twin ? 1u : 0u, winner_ms};
+                               winner.pair ? 1u : 0u, winner.twin, winner_ms};
     return TVS_OK;
 }
 

@@ -36,6 +36,13 @@ namespace tvs {
 namespace {
 
 constexpr uint32_t kBlock = 256;
+// Waves per SIMD the int16 eight-band walk (two pairs per lane, both
+// directions) is compiled for: left alone it takes ~98 VGPRs (four waves);
+// held to five waves it fits in 81 with no spill and runs ~2-3 % faster
+// than the four-band walk on 1201 x 1201 (same box), where with six it only
+// ties and with seven it spills.  The f32 form (two dword loads per pair)
+// spills at five and is left alone.
+constexpr int kQuadWaves = 5;
 
 __global__ __launch_bounds__(kBlock) void walk_plain_kernel(
     const float *__restrict__ elev, const int32_t *__restrict__ deltas,
@@ -605,6 +612,139 @@ __device__ __forceinline__ void walk_twin(const TileDesc &d, uint32_t rel, bool 
     }
 }
 
+// step_update with the surface weight in a vector register.
+__device__ __forceinline__ void step_update_v(float angle, float w, uint64_t lanes,
+                                              float &max_angle, float &surface) {
+    asm("v_cmpx_gt_f32 vcc, %[a], %[m]\n\t"
+        "v_add_f32 %[s], %[w], %[s]\n\t"
+        "v_mov_b32 %[m], %[a]\n\t"
+        "s_mov_b64 exec, %[e]"
+        : [m] "+v"(max_angle), [s] "+v"(surface)
+        : [a] "v"(angle), [w] "v"(w), [e] "s"(lanes)
+        : "vcc");
+}
+
+// A wave-uniform value copied into a vector register, so that the
+// instructions reading it take no scalar operand (two instructions that
+// each read a scalar register never issue together).
+__device__ __forceinline__ float to_vgpr(float x) {
+    float r;
+    asm("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
+    return r;
+}
+
+// Two pairs of adjacent points per lane, (ta, ta + 1) of one row and (tb,
+// tb + 1) of the next, in both directions: eight bands share every step's
+// constants, which are copied into vector registers once per step (four
+// moves for eight bands) so that the step's arithmetic reads no scalar
+// register.  Bands: a, a + 1, b, b + 1 forward, then the same backward.
+// `active_b` false: the lane's second pair duplicates the first (no row
+// below it) and is not written.  Regular steps only use the fast divides
+// (DIV 2 or 4); the zero tail is skipped as in walk_pair.
+template <typename H, int DIV, int UNROLL>
+__device__ __forceinline__ void walk_quad(const TileDesc &d, uint32_t rel, uint32_t ta,
+                                          uint32_t tb, bool active_a, bool active_b) {
+    static_assert(DIV == 2 || DIV == 4, "walk_quad takes the fast divides");
+    constexpr int NB = 8;
+    const H *elev = static_cast<const H *>(d.elev);
+    const uint32_t slot = d.first_slot + rel;
+    const uint32_t pov_a = (ta / d.T + d.L) * d.w + (ta % d.T + d.L);
+    const uint32_t pov_b = (tb / d.T + d.L) * d.w + (tb % d.T + d.L);
+    const uint32_t bytes_a = pov_a * static_cast<uint32_t>(sizeof(H));
+    const uint32_t bytes_b = pov_b * static_cast<uint32_t>(sizeof(H));
+    float h0[4];
+    h0[0] = static_cast<float>(global_ptr(elev)[pov_a]) + d.obs;
+    h0[1] = static_cast<float>(global_ptr(elev)[pov_a + 1u]) + d.obs;
+    h0[2] = static_cast<float>(global_ptr(elev)[pov_b]) + d.obs;
+    h0[3] = static_cast<float>(global_ptr(elev)[pov_b + 1u]) + d.obs;
+    const uint32_t head = const_ptr(d.meta)[slot].head;
+    const uint32_t n_fast = const_ptr(d.meta)[slot].n_fast;
+    const uint32_t tail_start = const_ptr(d.meta)[slot].tail_start;
+    const uint32_t bias = const_ptr(d.meta)[slot].bias;
+    const __amdgpu_buffer_rsrc_t raw = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<H *>(elev - bias), 0,
+        static_cast<int>((bias + d.span) * sizeof(H)), kHeightRsrcFlags<float>);
+
+    const size_t tb_ = static_cast<size_t>(slot) * d.stride;
+    const TVS_CONST uint32_t *__restrict__ pos_f = const_ptr(d.pos_f + tb_);
+    const TVS_CONST uint32_t *__restrict__ pos_b = const_ptr(d.pos_b + tb_);
+    const TVS_CONST float *__restrict__ dist = const_ptr(d.dist + tb_);
+    const TVS_CONST float *__restrict__ rcp = const_ptr(d.rcp + tb_);
+    const TVS_CONST float *__restrict__ rcp_lo = const_ptr(d.rcp_lo + tb_);
+    const TVS_CONST float *__restrict__ curv = const_ptr(d.curv + tb_);
+    const TVS_CONST float *__restrict__ wt = const_ptr(d.wt + tb_);
+    const uint64_t lanes = __builtin_amdgcn_read_exec();
+
+    float max_angle[NB], surface[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        max_angle[b] = kMaxAngle;
+        surface[b] = 0.0f;
+    }
+    auto load8 = [&](uint32_t pf, uint32_t pb, float (&h)[NB]) {
+        load_pair<H>(raw, bytes_a, pf, h[0], h[1]);
+        load_pair<H>(raw, bytes_b, pf, h[2], h[3]);
+        load_pair<H>(raw, bytes_a, pb, h[4], h[5]);
+        load_pair<H>(raw, bytes_b, pb, h[6], h[7]);
+    };
+    // One step with the IEEE divide (fast_k false) or the sector's divide.
+    auto single = [&](uint32_t k, bool fast_k) {
+        float h[NB];
+        load8(pos_f[k], pos_b[k], h);
+        const float dk = dist[k];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const float dh = h[b] - h0[b & 3];
+            const float q = fast_k ? step_div<DIV>(dh, dk, rcp[k], rcp_lo[k]) : dh / dk;
+            step_update(q - curv[k], wt[k], lanes, max_angle[b], surface[b]);
+        }
+    };
+    uint32_t k = 0;
+    for (; k < head; ++k) single(k, false);
+    const uint32_t fast_end = (n_fast - head) - (n_fast - head) % UNROLL;
+    const TVS_CONST uint32_t *__restrict__ fpos_f = pos_f + head;
+    const TVS_CONST uint32_t *__restrict__ fpos_b = pos_b + head;
+    const TVS_CONST float *__restrict__ fdist = dist + head;
+    const TVS_CONST float *__restrict__ frcp = rcp + head;
+    const TVS_CONST float *__restrict__ frcp_lo = rcp_lo + head;
+    const TVS_CONST float *__restrict__ fcurv = curv + head;
+    const TVS_CONST float *__restrict__ fwt = wt + head;
+    for (uint32_t j = 0; j < fast_end; j += UNROLL) {
+        float h[UNROLL][NB];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) load8(fpos_f[j + u], fpos_b[j + u], h[u]);
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            const float dk = DIV == 4 ? to_vgpr(fdist[j + u]) : 0.0f;
+            const float y_hi = to_vgpr(frcp[j + u]);
+            const float y_lo = to_vgpr(frcp_lo[j + u]);
+            const float c = to_vgpr(fcurv[j + u]);
+            const float w = to_vgpr(fwt[j + u]);
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const float dh = h[u][b] - h0[b & 3];
+                step_update_v(step_div<DIV>(dh, dk, y_hi, y_lo) - c, w, lanes, max_angle[b],
+                              surface[b]);
+            }
+        }
+    }
+    k += fast_end;
+    for (; k < tail_start; ++k) single(k, k < n_fast);
+    TVS_GLOBAL float *out = global_ptr(d.partial) + static_cast<size_t>(2u * rel) * d.N;
+    if (active_a) {
+        out[ta] = surface[0];
+        out[ta + 1u] = surface[1];
+        out[d.N + ta] = surface[4];
+        out[d.N + ta + 1u] = surface[5];
+    }
+    if (active_b) {
+        out[tb] = surface[2];
+        out[tb + 1u] = surface[3];
+        out[d.N + tb] = surface[6];
+        out[d.N + tb + 1u] = surface[7];
+    }
+}
+
 // Index of the tile whose [begin[i], begin[i+1]) holds x (wave-uniform).
 __device__ __forceinline__ uint32_t find_tile(const uint32_t *__restrict__ begin,
                                               uint32_t n_tiles, uint32_t x) {
@@ -646,7 +786,7 @@ __device__ __forceinline__ int divide_of(const TileDesc &d, uint32_t rel) {
 // loads of a whole wave are either all 4-byte aligned or all at 2 mod 4,
 // never mixed (a wave whose lanes span two rows would otherwise pay the
 // misaligned rate at every step).
-template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PAIR, bool TWIN,
+template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PAIR, int TWIN,
           bool IEEE>
 __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
                                           const uint32_t *__restrict__ item_begin,
@@ -662,7 +802,37 @@ __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
         ob = const_ptr(d.order)[(d.first_slot + rel) * d.blocks_per_pass + ob];
     const bool backward = !PAIR && (pass & 1u) != 0;
     const int div = IEEE ? 0 : divide_of(d, rel);
-    if (TWIN) {
+    if (TWIN == 2) {
+        // Lane groups: the column pairs of rows 2r and 2r + 1 (the last row
+        // alone when T is odd), each pair as in TWIN == 1.
+        const uint32_t half = d.T >> 1;
+        const uint32_t n_groups = twin_lane_groups(d.T, 2u);
+        const uint32_t group_blocks = (n_groups + BLOCK - 1) / BLOCK;
+        if (ob < group_blocks) {
+            const uint32_t q_raw = ob * BLOCK + threadIdx.x;
+            const bool active = q_raw < n_groups;
+            const uint32_t q = active ? q_raw : n_groups - 1u;
+            const uint32_t rp = q / half;
+            const uint32_t col = q - rp * half;
+            const uint32_t ra = 2u * rp;
+            const bool has_b = ra + 1u < d.T;
+            const uint32_t rb = has_b ? ra + 1u : ra;
+            const uint32_t ta = ra * d.T + 2u * col + (d.w & ra & 1u);
+            const uint32_t tb = rb * d.T + 2u * col + (d.w & rb & 1u);
+            if (!IEEE && div == 2) {
+                walk_quad<H, 2, UNROLL>(d, rel, ta, tb, active, active && has_b);
+            } else if (!IEEE && div == 4) {
+                walk_quad<H, 4, UNROLL>(d, rel, ta, tb, active, active && has_b);
+            } else {
+                // IEEE divide: the two pairs one after the other, at one
+                // step per iteration (the unrolled IEEE loop of eight bands
+                // would hold too many registers).
+                walk_twin<H, 0, 1, true>(d, rel, false, ta, active);
+                walk_twin<H, 0, 1, true>(d, rel, false, tb, active && has_b);
+            }
+            return;
+        }
+    } else if (TWIN) {
         const uint32_t half = d.T >> 1;
         const uint32_t n_pairs = d.T * half;
         const uint32_t twin_blocks = (n_pairs + BLOCK - 1) / BLOCK;
@@ -685,7 +855,7 @@ __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
     }
     // One point per lane: consecutive points, or (TWIN) each row's left-over
     // column.
-    const uint32_t lb = TWIN ? ob - (d.T * (d.T >> 1) + BLOCK - 1) / BLOCK : ob;
+    const uint32_t lb = TWIN ? ob - (twin_lane_groups(d.T, TWIN) + BLOCK - 1) / BLOCK : ob;
     const uint32_t i_raw = lb * BLOCK + threadIdx.x;
     const uint32_t n_single = TWIN ? d.T : d.N;
     const bool active = i_raw < n_single;
@@ -721,8 +891,8 @@ __device__ __forceinline__ void walk_item(const TileDesc *__restrict__ tiles,
 // share of every sector, and (with the directional block order) its share
 // is a strip along the sector's band direction.
 template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PERSIST, bool PAIR,
-          bool TWIN, bool IEEE>
-__global__ __launch_bounds__(BLOCK) void walk_tiles_kernel(
+          int TWIN, bool IEEE>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TWIN == 2 && sizeof(H) == 2 ? kQuadWaves : 1))) void walk_tiles_kernel(
     const TileDesc *__restrict__ tiles, const uint32_t *__restrict__ item_begin,
     uint32_t n_tiles, uint32_t n_items, uint32_t items_per_xcd, uint32_t pass_items,
     uint32_t pass_chunk) {
@@ -936,7 +1106,7 @@ uint32_t persistent_blocks(uint32_t block) {
 // `n_items` counts single-direction items; a PAIR launch has half as many
 // (every tile's count is even).
 template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PERSIST, bool PAIR,
-          bool TWIN, bool IEEE>
+          int TWIN, bool IEEE>
 hipError_t launch_tiles_v(hipStream_t s, const TileDesc *tiles, const uint32_t *item_begin,
                           uint32_t n_tiles, uint32_t n_items, bool xcd_contiguous,
                           uint32_t pass_items) {
@@ -958,7 +1128,7 @@ hipError_t launch_tiles_v(hipStream_t s, const TileDesc *tiles, const uint32_t *
 }
 
 template <typename H, bool CAPTURE, int BLOCK, int UNROLL, bool PIPE, bool PERSIST, bool PAIR,
-          bool TWIN>
+          int TWIN>
 hipError_t launch_tiles_t(hipStream_t s, const TileDesc *tiles, const uint32_t *item_begin,
                           uint32_t n_tiles, uint32_t n_items, bool xcd_contiguous,
                           bool ieee_only, uint32_t pass_items) {

@@ -111,7 +111,7 @@ class WalkShape(ctypes.Structure):
                 "directional": self.xcd_contiguous == 2,
                 "pipeline": bool(self.pipeline), "persistent": bool(self.persistent),
                 "both_directions": bool(self.both_directions),
-                "two_points": bool(self.two_points), "ms": self.ms}
+                "two_points": int(self.two_points), "ms": self.ms}
 
 
 class TvsError(RuntimeError):
@@ -397,7 +397,7 @@ class Engine:
 
     def set_walk_shape(self, block: int, unroll: int, xcd_contiguous: bool = True,
                        pipeline: bool = False, persistent: bool = False,
-                       both_directions: bool = False, two_points: bool = False) -> None:
+                       both_directions: bool = False, two_points: int = 0) -> None:
         self._c(load().tvs_set_walk_shape(self._h, ctypes.byref(
             WalkShape(block, unroll, int(xcd_contiguous), int(pipeline), int(persistent),
                       int(both_directions), int(two_points), 0.0))))
@@ -510,7 +510,7 @@ class Batch:
 
     def set_walk_shape(self, block: int, unroll: int, xcd_contiguous: bool = False,
                        pipeline: bool = False, persistent: bool = False,
-                       both_directions: bool = False, two_points: bool = False) -> None:
+                       both_directions: bool = False, two_points: int = 0) -> None:
         self._c(load().tvs_batch_set_walk_shape(self._h, ctypes.byref(
             WalkShape(block, unroll, int(xcd_contiguous), int(pipeline), int(persistent),
                       int(both_directions), int(two_points), 0.0))))
