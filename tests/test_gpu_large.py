@@ -1,7 +1,8 @@
 """The 3601 x 3601 tile at 30 m, LoS 40 km (1333 steps per band), through
-the C ABI.  The oracle checks band data and whole sectors on a sample; the
-full 180-sector run is checked by the batched walk (tuned two-point shapes,
-short divide) equalling the plain walk bit for bit."""
+the C ABI.  The oracle checks band data of all 180 sectors, whole sectors
+walked alone in three launch shapes, and the cumulative surfaces after
+every 30 sectors up to the full 180-sector pass; the full run is also
+checked as batched = plain."""
 import os
 import sys
 
@@ -86,3 +87,60 @@ def test_large_full_run_batched_equals_plain(large):
     np.testing.assert_array_equal(e.surfaces().view(np.uint32), plain.view(np.uint32))
     assert e.short_divide() == (True, 180)
     assert np.all(plain > 0)
+
+
+# ---- all 180 sectors against the oracle ------------------------------------
+GROUP = 30
+_prefix = {}   # end sector -> oracle cumulative surfaces after sectors [0, end)
+
+
+@pytest.fixture(scope="module")
+def large_all_bands(large, oracle):
+    """The oracle's band data for all 180 sectors (one sector per task)."""
+    from concurrent.futures import ThreadPoolExecutor
+    wl, dem, _ = large
+    note("large-3601 oracle band data, 180 sectors")
+    with ThreadPoolExecutor(THREADS) as ex:
+        parts = list(ex.map(lambda s: oracle.band_data(dem, s), range(180)))
+    return np.stack([p[0] for p in parts]), np.stack([p[1] for p in parts])
+
+
+def test_large_band_data_all_sectors(large, large_all_bands):
+    wl, dem, e = large
+    deltas, dists = large_all_bands
+    for s in range(180):
+        got_d, got_x = e.get_band(s)
+        np.testing.assert_array_equal(got_d, deltas[s], err_msg=f"deltas, sector {s}")
+        np.testing.assert_array_equal(got_x.view(np.uint32), dists[s].view(np.uint32),
+                                      err_msg=f"distances, sector {s}")
+
+
+def _oracle_prefix(oracle, wl, dem, bands, end):
+    """Oracle cumulative surfaces after sectors [0, end), extended from the
+    longest prefix already walked (each prefix is a copy)."""
+    if end in _prefix:
+        return _prefix[end]
+    start = max([k for k in _prefix if k < end], default=0)
+    cum = _prefix[start].copy() if start else np.zeros(dem.computable_points_count, np.float32)
+    deltas, dists = bands
+    note(f"large-3601 oracle walk, sectors {start}..{end - 1}")
+    oracle.walk_sectors(dem, wl.elevations.astype(np.float32), wl.observer_height,
+                        deltas[start:end], dists[start:end], n_threads=THREADS, cumulative=cum)
+    _prefix[end] = cum
+    return cum
+
+
+@pytest.mark.parametrize("end", range(GROUP, 181, GROUP))
+def test_large_all_sectors_equal_oracle(large, large_all_bands, oracle, end):
+    """The batched walk over sectors [0, end) in one launch, in the
+    directional block order the tuner picks for this tile, equals the
+    oracle's cumulative surfaces after the same sectors (end = 180: the
+    whole pass, compute.rs:101-115)."""
+    wl, dem, e = large
+    e.set_walk_shape(64, 8, 2, both_directions=True, two_points=1)
+    e.reset_surfaces()
+    e.run(0, end, "batched")
+    got = e.surfaces()
+    want = _oracle_prefix(oracle, wl, dem, large_all_bands, end)
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32),
+                                  err_msg=f"sectors 0..{end - 1}")
