@@ -121,7 +121,12 @@ def cpu_baseline(wl, sectors: int, threads: int, keep_surfaces: bool = False):
     return (res, surfaces) if keep_surfaces else res
 
 
-def measure_large(steps: int, warmup: int, cpu: bool = False) -> dict:
+def parse_shape(text: str) -> tuple:
+    """B,U,X,P,S,D,T (see --shape) as the arguments of set_walk_shape."""
+    return tuple(int(v) for v in text.split(","))
+
+
+def measure_large(steps: int, warmup: int, cpu: bool = False, shape_arg=None) -> dict:
     """3601 x 3601 at 30 m, LoS 40 km (1333 steps per band), int16.  With
     `cpu`, also the oracle walk on one core over every 10th sector
     (extrapolated to 180 by the executed-step rate) and on every host core
@@ -135,7 +140,11 @@ def measure_large(steps: int, warmup: int, cpu: bool = False) -> dict:
         e.upload(wl.elevations)
         e.compute_bands(0, 180)
         bands_ms = e.timing().bands_ms
-        shape, _ = e.autotune(0, 180, reps=3)
+        if shape_arg:
+            e.set_walk_shape(*shape_arg)
+            shape = e.walk_shape()
+        else:
+            shape, _ = e.autotune(0, 180, reps=3)
         ms, kernel_ms, tm = time_method(e, "batched", steps, warmup)
         e.compact_rings(45)                       # allocates; another sector
         t0 = time.perf_counter()
@@ -185,7 +194,7 @@ def cpu_tiles(tiles, sectors: int, threads: int, gpu_surfaces=None) -> dict:
     return res
 
 
-def measure_tiles(steps: int, warmup: int, cpu: bool = False) -> dict:
+def measure_tiles(steps: int, warmup: int, cpu: bool = False, shape_arg=None) -> dict:
     """64 uneven tiles (sides from {257, 401, 601, 801, 1201}, 30/90 m), all
     180 sectors of all tiles in one walk launch + one reduction, replayed
     from a hipGraph.  With `cpu`, also the oracle walk on one core over
@@ -200,7 +209,13 @@ def measure_tiles(steps: int, warmup: int, cpu: bool = False) -> dict:
             b.upload(i, t.elevations)
         b.compute_bands()
         bands_ms = b.timing().bands_ms
-        shape, _ = b.autotune(reps=3)
+        if shape_arg:
+            b.set_walk_shape(*shape_arg)
+            shape = dict(zip(("block", "unroll", "xcd_contiguous", "pipeline", "persistent",
+                              "both_directions", "two_points"), shape_arg))
+            shape["xcd_contiguous"], shape["directional"] = bool(shape_arg[2]), shape_arg[2] == 2
+        else:
+            shape, _ = b.autotune(reps=3)
         log(f"tiles-64 tuned shape: block {shape['block']}, unroll {shape['unroll']}, "
             f"xcd {shape['xcd_contiguous']}, directional {shape['directional']}, both directions {shape['both_directions']}, "
             f"two points {shape['two_points']}")
@@ -266,7 +281,8 @@ def measure_next_rows(wl, shape: dict) -> dict:
                             wl.observer_height, 0) as e:
             e.upload(wl.elevations)
             e.compute_bands(0, 180)
-            e.set_walk_shape(shape["block"], shape["unroll"], shape["xcd_contiguous"],
+            e.set_walk_shape(shape["block"], shape["unroll"],
+                             2 if shape["directional"] else int(shape["xcd_contiguous"]),
                              shape["pipeline"], shape["persistent"], shape["both_directions"],
                              shape["two_points"])
             t0 = time.perf_counter()
@@ -444,6 +460,9 @@ def main() -> int:
     ap.add_argument("--cpu-tiles", action="store_true",
                     help="also time the CPU oracle on the 64 tiles (about a minute)")
     ap.add_argument("--cpu-threads", type=int, default=1)
+    ap.add_argument("--only", choices=["large-3601", "tiles-64"],
+                    help="only time this workload (as in the bench line's `extra`) and print "
+                         "its JSON")
     ap.add_argument("--band-paths", action="store_true",
                     help="only time band data by selection and by full sorts, checked equal")
     ap.add_argument("--copy-calibration", type=int, metavar="MIB",
@@ -457,6 +476,12 @@ def main() -> int:
         return copy_calibration(args.copy_calibration)
     if args.band_paths:
         return band_paths(args.workload)
+    if args.only:
+        fn = measure_large if args.only == "large-3601" else measure_tiles
+        print(json.dumps(fn(args.steps, args.warmup,
+                            shape_arg=parse_shape(args.shape) if args.shape else None)),
+              flush=True)
+        return 0
     if args.summarize_pmc:
         a = args.summarize_pmc
         return summarize_pmc(a[0], a[1], a[2], a[3], int(a[4]), a[5])
@@ -485,8 +510,7 @@ def main() -> int:
     real_point_steps = 180 * 2 * g.computable_points_count * g.real_steps
 
     if args.shape:
-        b, u, x, p, ps, dd, tw = (int(v) for v in args.shape.split(","))
-        e.set_walk_shape(b, u, x, p, ps, dd, tw)
+        e.set_walk_shape(*parse_shape(args.shape))
     shape = e.walk_shape()
     if not args.no_tune and not args.shape:
         shape, table = e.autotune(0, 180, reps=3)
