@@ -308,6 +308,20 @@ def measure_next_rows(wl, shape: dict) -> dict:
             bands = 2 * e.n_points
             res["ring_events_one_sector"] = int(events.size)
             res["ring_export_bytes"] = int(4 * (counts.size + events.size) + 8 * bands)
+            # One observer's 360 bands, one wavefront per band; its sector-46
+            # bands carry the same events as that sector's export.
+            from tvs_amd import rings as ring_tools
+            t_obs = e.n_points // 2 + 17
+            e.observer_walk(t_obs)                    # allocates
+            t0 = time.perf_counter()
+            oc, _, oev = e.observer_walk(t_obs)
+            res["observer_walk_ms_180_sectors"] = (time.perf_counter() - t0) * 1000.0
+            coff, ooff = ring_tools.offsets(counts), ring_tools.offsets(oc)
+            for direction in (0, 1):
+                k, b = t_obs + direction * e.n_points, 2 * 46 + direction
+                assert counts[k] == oc[b] and np.array_equal(
+                    events[coff[k]:coff[k] + counts[k] - 1], oev[ooff[b]:ooff[b] + oc[b] - 1]), \
+                    "observer walk against the sector export"
             res["ring_fixed_slot_bytes"] = int(4 * bands * tvs_amd.ring_count_per_band(
                 wl.rings_per_km, wl.max_line_of_sight))
             e.run(0, 180, "batched")
@@ -341,7 +355,8 @@ def measure_next_rows(wl, shape: dict) -> dict:
     log(f"next rows: .bt write {res['bt_write_ms']:.1f} ms / read {res['bt_read_ms']:.1f} ms, "
         f"cache save {res['cache_save_ms_180']:.1f} ms / load {res['cache_load_ms_180']:.1f} ms "
         f"(180 sectors), ring export {res['ring_export_ms_one_sector']:.1f} ms "
-        f"({res['ring_events_one_sector']} events), PNG {res['png_ms']:.1f} ms, "
+        f"({res['ring_events_one_sector']} events), one observer's 360 bands "
+        f"{res['observer_walk_ms_180_sectors']:.2f} ms, PNG {res['png_ms']:.1f} ms, "
         f"host to host {res['host_to_host_ms']:.1f} ms, "
         f"CLI end to end {res['cli_end_to_end_ms']:.0f} ms")
     return res

@@ -261,6 +261,24 @@ tvs_status tvs_compact_rings(tvs_ctx *ctx, uint32_t sector, uint32_t *counts,
                              size_t n_counts, uint32_t *events,
                              size_t events_capacity, uint64_t *n_events);
 
+/* One observer's bands: the two kernel ids of observer `tvs_id` (t and
+ * t + N, kernel.rs:117-131) in each sector of [first, first+count), walked
+ * one wavefront per band (64 consecutive steps per pass, prefix maximum
+ * across the lanes, ballots for the ring-event slots), every step as the
+ * reference (IEEE divide, zero tail included).  Bands in order (sector,
+ * forward then backward): `counts[b]` = the count the reference stores at
+ * slot 0 of the band (kernel.rs:243: 1 + events), `band_surfaces[b]` (may
+ * be NULL) = the band's f32 surface (kernel.rs:216), `events` = every
+ * band's events after its PoV slot, band after band (as tvs_compact_rings
+ * packs them); call with events == NULL to get the counts and `*n_events`
+ * only.  No counterpart in the reference, which rebuilds one observer's
+ * viewshed from the ring data of every band (output/ascii.rs:24-104).
+ * Synchronous; surfaces and ring records of runs are not touched. */
+tvs_status tvs_observer_walk(tvs_ctx *ctx, uint32_t tvs_id, uint32_t first_sector,
+                             uint32_t sector_count, uint32_t *counts,
+                             float *band_surfaces, uint32_t *events,
+                             size_t events_capacity, uint64_t *n_events);
+
 /* Ring events dropped by the last run because a band had more than R-1. */
 tvs_status tvs_ring_overflow_count(tvs_ctx *ctx, uint64_t *out);
 

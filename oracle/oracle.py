@@ -234,6 +234,28 @@ def compute_run(dem: Dem, elevations: np.ndarray, observer_height: float = 1.8,
     return cum, ring, int(spill.value)
 
 
+def kernel_band(dem: Dem, elevations: np.ndarray, observer_height: float,
+                deltas: np.ndarray, dists: np.ndarray, kernel_id: int):
+    """kernel.rs:71-256 for one kernel id over one sector's band data: the
+    band surface (added to a zero cumulative surface, so exactly the band's)
+    and its ring record [count, PoV, events...] with room for every event."""
+    elev = np.ascontiguousarray(elevations, np.float32)
+    deltas = np.ascontiguousarray(deltas, np.int32)
+    dists = np.ascontiguousarray(dists, np.float32)
+    n = deltas.size
+    R = n + 3
+    c = _Constants(2 * dem.computable_points_count, dem.max_los_as_points, dem.width,
+                   dem.tvs_width, observer_height, R)
+    cum = np.zeros(dem.computable_points_count, np.float32)
+    ring = np.zeros((2 * dem.computable_points_count) * R, np.uint32)
+    _check(lib().tvso_kernel(kernel_id, ctypes.byref(c), _ptr(elev), elev.size, _ptr(dists),
+                             _ptr(deltas), n, _ptr(cum), _ptr(ring), ring.size, None),
+           f"kernel {kernel_id}")
+    t = kernel_id % dem.computable_points_count
+    rec = ring[kernel_id * R:(kernel_id + 1) * R]
+    return cum[t], rec[:int(rec[0]) + 1]
+
+
 def walk_sectors(dem: Dem, elevations: np.ndarray, observer_height: float,
                  deltas: np.ndarray, dists: np.ndarray, rings_per_band: int = 0,
                  rings: bool = False, n_threads: int = 1,

@@ -501,6 +501,7 @@ tvs_status tvs_destroy(tvs_ctx *c) {
     dev_free(c->d_ring_offsets);
     dev_free(c->d_ring_events);
     dev_free(c->d_order);
+    dev_free(c->d_obs);
     dev_free(c->d_axes);
     dev_free(c->d_select_status);
     dev_free(c->d_begin);

@@ -252,6 +252,8 @@ struct tvs_ctx {
     uint64_t *d_ring_offsets = nullptr;   // u64[2N + 1 + tiles]: offsets, total, scan scratch
     uint32_t *d_ring_events = nullptr;
     uint32_t *d_order = nullptr;          // directional item order (see WalkConfig)
+    uint32_t *d_obs = nullptr;            // tvs_observer_walk: counts, surfaces, event slots
+    size_t obs_cap = 0;                   // u32 entries of d_obs
     uint64_t order_key = 0;               // (T, w, block, twin) it was built for; 0: none
     size_t ring_events_cap = 0;
     // Changes with every upload and band change; the compact ring export

@@ -150,6 +150,8 @@ def load() -> ctypes.CDLL:
         "tvs_compute_axes": (st, [vp, ctypes.c_uint32, vp, vp, vp]),
         "tvs_run_sectors": (st, [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]),
         "tvs_run_sectors_noalloc": (st, [vp, ctypes.c_uint32, ctypes.c_uint32]),
+        "tvs_observer_walk": (st, [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, vp, vp,
+                                   vp, ctypes.c_size_t, P(ctypes.c_uint64)]),
         "tvs_run_sector_with_band": (st, [vp, vp, vp, ctypes.c_size_t, vp, ctypes.c_size_t]),
         "tvs_reset_surfaces": (st, [vp]),
         "tvs_read_surfaces": (st, [vp, vp, ctypes.c_size_t]),
@@ -389,6 +391,21 @@ class Engine:
         self._c(load().tvs_compact_rings(self._h, sector, _ptr(counts), bands, _ptr(events),
                                          events.size, ctypes.byref(total)))
         return counts[:bands], events[:int(total.value)]
+
+    def observer_walk(self, tvs_id: int, first: int = 0, count: int = 180):
+        """tvs_observer_walk: one observer's bands (sector after sector,
+        forward then backward), one wavefront per band.  Returns (counts
+        u32[2*count], band surfaces f32[2*count], events u32[total])."""
+        bands = 2 * count
+        counts = np.empty(max(bands, 1), np.uint32)
+        surf = np.empty(max(bands, 1), np.float32)
+        total = ctypes.c_uint64(0)
+        self._c(load().tvs_observer_walk(self._h, tvs_id, first, count, _ptr(counts), _ptr(surf),
+                                         None, 0, ctypes.byref(total)))
+        events = np.empty(max(int(total.value), 1), np.uint32)
+        self._c(load().tvs_observer_walk(self._h, tvs_id, first, count, _ptr(counts), _ptr(surf),
+                                         _ptr(events), events.size, ctypes.byref(total)))
+        return counts[:bands], surf[:bands], events[:int(total.value)]
 
     def ring_overflow(self) -> int:
         v = ctypes.c_uint64(0)

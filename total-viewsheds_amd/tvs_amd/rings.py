@@ -60,12 +60,26 @@ def viewshed_rings(geometry, pov_id: int, sectors) -> list[tuple[int, int]]:
     return pairs
 
 
-def ascii_viewshed(geometry, pov_id: int, sectors) -> list[str]:
+def observer_rings(pov_id: int, counts: np.ndarray, events: np.ndarray) -> list[tuple[int, int]]:
+    """Every (opening, closing) pair of one observer's bands as
+    `Engine.observer_walk` returns them (counts and packed events)."""
+    off = offsets(counts)
+    pairs = []
+    for k in range(counts.size):
+        ids = [pov_id] + [int(x) for x in events[off[k]:off[k] + int(counts[k]) - 1]]
+        pairs.extend((ids[2 * i], ids[2 * i + 1]) for i in range(len(ids) // 2))
+    return pairs
+
+
+def ascii_viewshed(geometry, pov_id: int, sectors=None, pairs=None) -> list[str]:
     """One observer's viewshed as text rows: '+' where a ring opens, '-'
     where one closes, '±' both, 'o' the observer (the rendering of
-    output/ascii.rs:88-104)."""
+    output/ascii.rs:88-104), from the sectors' compact rings or from ready
+    (opening, closing) pairs."""
     cells = ["."] * geometry.dem_size
-    for opening, closing in viewshed_rings(geometry, pov_id, sectors):
+    if pairs is None:
+        pairs = viewshed_rings(geometry, pov_id, sectors)
+    for opening, closing in pairs:
         cells[opening] = "+" if cells[opening] == "." else "±"
         cells[closing] = "-" if cells[closing] == "." else "±"
         if opening == closing:
