@@ -106,3 +106,39 @@ def test_heatmap_png(tmp_path, shape):
     rng_ = np.float32(1.0) if mx - mn == 0 else np.float32(mx - mn)
     want = np.clip((data - mn) / rng_ * np.float32(255.0), 0, 255).astype(np.uint8)
     np.testing.assert_array_equal(read_png_grey(str(p)), want)
+
+
+def _png_of(tmp_path, data):
+    p = tmp_path / "k.png"
+    bt.heatmap_png(str(p), np.ascontiguousarray(data, np.float32))
+    return read_png_grey(str(p))
+
+
+def test_heatmap_known_answers(tmp_path):
+    """Cases the arithmetic fixes (output/png.rs:5-37), not a restated
+    formula: with a power-of-two range every step is exact, so a pixel is
+    floor(255 (v - min) / range); the minimum maps to 0 and the maximum to
+    255 (range / range = 1); constant data is all 0 (range 1, v - min = 0);
+    the map is monotone."""
+    v = np.arange(0, 65, dtype=np.float64).reshape(5, 13) - 20.0   # range 64
+    np.testing.assert_array_equal(_png_of(tmp_path, v),
+                                  np.floor(255.0 * (v + 20.0) / 64.0).astype(np.uint8))
+    rng = np.random.default_rng(3)
+    data = (rng.random((40, 40)) * 1e4 - 3e3).astype(np.float32)
+    px = _png_of(tmp_path, data)
+    assert px.flat[np.argmin(data)] == 0 and px.flat[np.argmax(data)] == 255
+    order = np.argsort(data, axis=None, kind="stable")
+    assert np.all(np.diff(px.reshape(-1)[order].astype(int)) >= 0)
+    assert np.all(_png_of(tmp_path, np.full((3, 4), 7.5)) == 0)
+
+
+def test_heatmap_nan_and_infinities(tmp_path):
+    """IEEE arithmetic and Rust's float-to-int cast (NaN to 0, saturating)
+    fix these: a NaN pixel is 0 (f32::min/max skip it for the range); an
+    infinite value makes the range infinite, so finite pixels divide to 0
+    and the infinite one to inf - inf = NaN, hence 0."""
+    data = np.array([[0.0, 1.0], [np.nan, 2.0]], np.float32)
+    np.testing.assert_array_equal(_png_of(tmp_path, data), [[0, 127], [0, 255]])
+    for inf in (np.inf, -np.inf):
+        data = np.array([[0.0, 1.0], [inf, 2.0]], np.float32)
+        assert np.all(_png_of(tmp_path, data) == 0)

@@ -243,9 +243,14 @@ tvs_status tvs_write_heatmap_png(const char *path, const float *data, uint32_t w
     for (uint32_t r = 0; r < height; ++r) {
         raw.push_back(0);  // filter: none
         for (uint32_t x = 0; x < width; ++x) {
+            // png.rs:16-25: `((value - min) / range * 255.0).clamp(0.0, 255.0)
+            // as u8`.  A NaN stays NaN through clamp, and Rust's float-to-int
+            // `as` maps NaN to 0: test it before clamping (fmin/fmax would
+            // turn it into 255).
             float v = (data[static_cast<size_t>(r) * width + x] - mn) / range * 255.0f;
+            if (v != v) v = 0.0f;
             v = std::fmax(0.0f, std::fmin(255.0f, v));
-            raw.push_back(static_cast<unsigned char>(v != v ? 0.0f : v));
+            raw.push_back(static_cast<unsigned char>(v));
         }
     }
     // zlib stream of stored blocks.
