@@ -65,6 +65,24 @@ def test_noalloc_equals_the_allocating_run_on_two_calls():
         e.close()
 
 
+def test_noalloc_f32_heights():
+    """Non-integer f32 heights (the f32 kernels, the four-operation divide):
+    the same bit-exact agreement."""
+    small = demgen.small_201(seed=5)
+    small.elevations = small.elevations.astype(np.float32) + np.float32(0.25)
+    print("noalloc: small-201 + 0.25 m, f32 heights", flush=True)
+    ref, e = _engines(small)
+    try:
+        assert e.fast_divide()
+        ref.run(0, 180, "batched")
+        e.run_noalloc(0, 180)
+        _same(e.surfaces(), ref.surfaces(), "f32 heights, f32 sums")
+        _same(e.surfaces_f64(), ref.surfaces_f64(), "f32 heights, f64 sums")
+    finally:
+        ref.close()
+        e.close()
+
+
 def test_noalloc_refusals():
     small = demgen.small_201(seed=3)
     with tvs_amd.Engine(small.width, small.spacing, small.max_line_of_sight,
