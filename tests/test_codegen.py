@@ -143,8 +143,13 @@ def test_eight_band_walk_loop_shape(asm):
     assert valu(short) <= 272 + 8 and valu(split) <= 336 + 8, (valu(short), valu(split))
     assert scalar_operand_valu(short) <= 16 and scalar_operand_valu(split) <= 20
     four = unrolled_loops(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb1ELi1ELb0E"))
-    four_short = min((l for l in four if "v_div_scale" not in l), key=valu)
-    assert scalar_operand_valu(four_short) >= valu(four_short) // 2 - 8
+    four_short = min((l for l in four if "v_div_scale" not in l and
+                      "buffer_load_format_x" not in l), key=valu)
+    assert valu(four_short) <= 8 * 32 + 8 + 8
+    assert scalar_operand_valu(four_short) >= valu(four_short) // 3
+    # The four-band loop copies the curvature term into a vector register
+    # once per step (8 per chunk), nothing else.
+    assert len(re.findall(r"v_mov_b32(?:_e32)?\s+v\d+, s\d+", four_short)) == 8
 
 
 def test_eight_band_kernel_holds_five_waves(asm):

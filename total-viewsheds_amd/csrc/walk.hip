@@ -194,6 +194,27 @@ __device__ __forceinline__ void pair_update(float angle_f, float angle_b, float 
         : "vcc");
 }
 
+// step_update with the surface weight in a vector register.
+__device__ __forceinline__ void step_update_v(float angle, float w, uint64_t lanes,
+                                              float &max_angle, float &surface) {
+    asm("v_cmpx_gt_f32 vcc, %[a], %[m]\n\t"
+        "v_add_f32 %[s], %[w], %[s]\n\t"
+        "v_mov_b32 %[m], %[a]\n\t"
+        "s_mov_b64 exec, %[e]"
+        : [m] "+v"(max_angle), [s] "+v"(surface)
+        : [a] "v"(angle), [w] "v"(w), [e] "s"(lanes)
+        : "vcc");
+}
+
+// A wave-uniform value copied into a vector register, so that the
+// instructions reading it take no scalar operand (two instructions that
+// each read a scalar register never issue together).
+__device__ __forceinline__ float to_vgpr(float x) {
+    float r;
+    asm("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
+    return r;
+}
+
 // Pointers read out of a TileDesc are generic; naming them global lets the
 // wave-uniform table reads become scalar loads (s_load_dwordx8) instead of
 // per-lane loads plus a readfirstlane loop for the scalar offset.
@@ -588,7 +609,15 @@ __device__ __forceinline__ void walk_twin(const TileDesc &d, uint32_t rel, bool 
             const float dk = fdist[j + u];
             const float y_hi = frcp[j + u];
             const float y_lo = frcp_lo[j + u];
-            const float c = fcurv[j + u];
+            // Two vector instructions issue in one quad-cycle only from two
+            // waves, and never when both read a scalar register: with four
+            // bands per lane the curvature term is copied into a vector
+            // register once per step (a quarter instruction per band-step),
+            // which leaves three of the step's eight instructions with a
+            // scalar operand instead of four.  Measured 1-1.5 % faster on
+            // 1201 x 1201 (same box); copying the reciprocal pair or the
+            // weight as well, or instead, gained less or lost.
+            const float c = BOTH ? to_vgpr(fcurv[j + u]) : fcurv[j + u];
             const float w = fwt[j + u];
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
@@ -610,27 +639,6 @@ __device__ __forceinline__ void walk_twin(const TileDesc &d, uint32_t rel, bool 
             out[d.N + 1] = surface[NB - 1];
         }
     }
-}
-
-// step_update with the surface weight in a vector register.
-__device__ __forceinline__ void step_update_v(float angle, float w, uint64_t lanes,
-                                              float &max_angle, float &surface) {
-    asm("v_cmpx_gt_f32 vcc, %[a], %[m]\n\t"
-        "v_add_f32 %[s], %[w], %[s]\n\t"
-        "v_mov_b32 %[m], %[a]\n\t"
-        "s_mov_b64 exec, %[e]"
-        : [m] "+v"(max_angle), [s] "+v"(surface)
-        : [a] "v"(angle), [w] "v"(w), [e] "s"(lanes)
-        : "vcc");
-}
-
-// A wave-uniform value copied into a vector register, so that the
-// instructions reading it take no scalar operand (two instructions that
-// each read a scalar register never issue together).
-__device__ __forceinline__ float to_vgpr(float x) {
-    float r;
-    asm("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
-    return r;
 }
 
 // Two pairs of adjacent points per lane, (ta, ta + 1) of one row and (tb,
