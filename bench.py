@@ -43,12 +43,15 @@ HBM_PEAK_GBPS = 8000.0
 SIMDS = 256 * 4
 CLOCK_HZ = 2.4e9
 VALU_ISSUE_PEAK_LANE_OPS = SIMDS * (2.0 / 4.0) * CLOCK_HZ * 64
-# Vector instructions per real band-step of the batched walk's unrolled
-# loop: sub, the divide (4 operations, or 2 with the verified short divide),
-# sub, cmpx, masked add, masked move, plus one conversion when two points
-# per lane share a dword load (tests/test_codegen.py holds the compiled
-# loops to these counts).
-VALU_PER_STEP = {"split": 9, "short": 7}
+# Vector instructions every real band-step of the batched walk's unrolled
+# loop executes: sub (height difference), the divide (4 operations, or 2
+# with the verified short divide), sub (curvature), compare, plus one
+# conversion when two points per lane share a dword load
+# (tests/test_codegen.py holds the compiled loops to these counts).  The
+# masked surface add and maximum move run only where a lane of the wave sees
+# the point (~2 % of a lane's band-steps, ~35 % of a wave's at 1201 x 1201)
+# and are not counted, so `achieved` is the work the walk cannot skip.
+VALU_PER_STEP = {"split": 7, "short": 5}
 
 
 def log(msg: str) -> None:

@@ -3,7 +3,8 @@ hipcc cross-compiles).  Guards the properties the kernel's speed rests on:
 8 height loads with scalar offsets per chunk (typed loads that convert the
 int16 heights, so no conversion instruction), the step tables read with
 scalar dwordx8 loads, the exec-masked visibility update, two FMAs fewer per
-step in the short-divide loop, and no divergent branch in the loop."""
+step in the short-divide loop, and the only branches in the loop those
+that skip a band's masked update when no lane of the wave sees the point."""
 import os
 import re
 import shutil
@@ -56,12 +57,21 @@ def valu(loop: str) -> int:
     return len(re.findall(r"^\s+v_", loop, re.M))
 
 
+def check_updates(loop: str, band_steps: int) -> None:
+    """One compare per band-step, each followed by the branch that skips the
+    band's masked update (exec set from vcc, add, move, exec restored) when
+    no lane sees the point; no other divergent branch."""
+    assert "s_cbranch_execz" not in loop and "v_cmpx" not in loop
+    assert len(re.findall(r"v_cmp_gt_f32\s+vcc", loop)) == band_steps
+    assert len(re.findall(r"v_cmp_gt_f32\s+vcc, v\d+, v\d+\n\s+s_cbranch_vccz", loop)) == band_steps
+    assert len(re.findall(r"s_mov_b64\s+exec, vcc", loop)) == band_steps
+
+
 def check_loop(loop: str, load: str, band_steps: int) -> None:
-    assert "s_cbranch_execz" not in loop and "s_cbranch_vccz" not in loop
+    check_updates(loop, band_steps)
     assert len(re.findall(load + r"\s+v\d+, v\d+, s\[\d+:\d+\], s\d+ offen", loop)) == band_steps
     assert "s_load_dwordx8" in loop
     assert "v_cvt_f32_i32" not in loop
-    assert len(re.findall(r"v_cmpx_gt_f32", loop)) == band_steps
     assert "scratch" not in loop and "buffer_store" not in loop
 
 
@@ -104,10 +114,9 @@ def test_two_point_walk_loop_shape(asm):
     twin = [l for l in loops if "v_div_scale" not in l and "buffer_load_dword" in l]
     assert len(twin) == 2
     for loop in twin:
-        assert "s_cbranch_execz" not in loop and "s_cbranch_vccz" not in loop
+        check_updates(loop, 16)
         assert len(re.findall(r"buffer_load_dword\s+v\d+, v\d+, s\[\d+:\d+\], s\d+ offen",
                               loop)) == 8
-        assert len(re.findall(r"v_cmpx_gt_f32", loop)) == 16
         assert len(re.findall(r"_sdwa", loop)) == 16
         assert "scratch" not in loop and "buffer_store" not in loop
     short, split = sorted(twin, key=valu)
@@ -134,10 +143,9 @@ def test_eight_band_walk_loop_shape(asm):
     fast = [l for l in loops if "v_div_scale" not in l and "buffer_load_format_x" not in l]
     assert len(fast) == 2
     for loop in fast:
-        assert "s_cbranch_execz" not in loop and "s_cbranch_vccz" not in loop
+        check_updates(loop, 32)
         assert len(re.findall(r"buffer_load_dword\s+v\d+, v\d+, s\[\d+:\d+\], s\d+ offen",
                               loop)) == 16
-        assert len(re.findall(r"v_cmpx_gt_f32", loop)) == 32
         assert "scratch" not in loop and "buffer_store" not in loop
     short, split = sorted(fast, key=valu)
     assert valu(short) <= 272 + 8 and valu(split) <= 336 + 8, (valu(short), valu(split))
