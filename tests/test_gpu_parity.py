@@ -114,6 +114,38 @@ def test_full_int16_range_bit_exact(oracle):
                                           err_msg=np.dtype(dtype).name)
 
 
+@pytest.mark.parametrize("terrain", ["flat", "spikes"])
+def test_update_branch_extremes(oracle, terrain):
+    """The batched walk skips a band's masked update when no lane of the
+    wave sees the point.  Flat terrain: every real step of every band is
+    visible (the angle -obs/d - d/12742000 rises with the distance), so no
+    update is ever skipped.  A field of 30000 m spikes on every third row
+    and column: nearly every band meets a spike within its first steps and
+    sees nothing after it, so nearly every update is skipped.  Both against
+    the oracle, bit for bit, in the default two-point shape and the
+    one-point shape (both directions per lane)."""
+    w, spacing, los = 121, 90.0, 3600
+    elev = np.zeros((w, w), dtype=np.int16)
+    if terrain == "spikes":
+        elev[::3, ::3] = 30000
+        elev[1::3, 2::3] = 120
+    elev = elev.reshape(-1)
+    note(f"update branch extremes ({terrain}) 121x121: oracle band data + walk")
+    dem = oracle.dem_new(w, spacing, los)
+    deltas, dists = oracle.band_data_all(dem, range(180))
+    cum, _ = oracle.walk_sectors(dem, elev.astype(np.float32), 1.65, deltas, dists,
+                                 n_threads=THREADS)
+    for shape in [(64, 8, True, False, False, True, 1), (128, 8, True, False, False, True, 0)]:
+        note(f"update branch extremes ({terrain}): batched, shape {shape}")
+        with tvs_amd.Engine(w, spacing, los, 5.0, 1.65, 0) as e:
+            e.upload(elev)
+            e.compute_bands(0, 180)
+            e.set_walk_shape(*shape)
+            e.run(0, 180, "batched")
+            np.testing.assert_array_equal(e.surfaces().view(np.uint32), cum.view(np.uint32),
+                                          err_msg=str(shape))
+
+
 @pytest.mark.parametrize("case", ["fraction", "tiny", "integer", "wide"])
 def test_fractional_heights_fast_divide(oracle, case):
     """Non-integer f32 heights walk with the four-operation divide when every
