@@ -615,6 +615,24 @@ def main() -> int:
     if os.path.exists(prof):
         with open(prof) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+    # Issue counters of the timed shape (rocprofv3 --pmc, one profiled run of
+    # this workload): every vector instruction issued, the masked updates
+    # included, and how busy the texture units are.
+    issue = None
+    prof = os.path.join(ROOT, "profiles", "walk_tiles_main-1201_issue.json")
+    timed_shape = (shape["block"], shape["unroll"], bool(shape["xcd_contiguous"]),
+                   bool(shape["directional"]), bool(shape["pipeline"]),
+                   bool(shape["persistent"]), bool(shape["both_directions"]),
+                   int(shape["two_points"]))
+    if (os.path.exists(prof) and wl.name == "main-1201"
+            and timed_shape == (64, 8, True, False, False, False, True, 1)):
+        with open(prof) as f:
+            pj = json.load(f)
+        issue = {"valu_per_band_step": pj["valu_per_band_step"],
+                 "issued_frac": pj["issued_frac_of_dual_issue_ceiling"],
+                 "dual_issue_quad_cycle_frac": pj["dual_issue_quad_cycle_frac"],
+                 "texture_units_busy_frac": pj["texture_units_busy_frac"],
+                 "source": "profiles/walk_tiles_main-1201_issue.json"}
 
     out = {
         "metric": METRIC,
@@ -671,6 +689,7 @@ def main() -> int:
                     "frac": (traffic / (b_kernel_ms / 1000.0) / 1e9 / HBM_PEAK_GBPS
                              if traffic else None)},
             "l2_served_gather_bytes_per_launch": gather_bytes,
+            "profiled_issue": issue,
         },
         "nominal_value": point_steps / (b_ms / 1000.0),
         "plain": plain,
