@@ -80,7 +80,11 @@ def time_method(e, method: str, steps: int, warmup: int):
     return wall * 1000.0 / steps, float(np.mean(kernel_ms)), e.timing()
 
 
-def cpu_baseline(wl, sectors: int, threads: int, keep_surfaces: bool = False):
+def cpu_baseline(wl, sectors: int, threads: int, keep_surfaces: bool = False,
+                 fp64: bool = False):
+    """The oracle's walk on a sample of sectors.  With `keep_surfaces` also
+    its cumulative f32 surfaces; with `fp64` as well the FP64 oracle walk over
+    the same band data: (tvs64, visibility flips, point-steps)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from concurrent.futures import ThreadPoolExecutor
@@ -121,7 +125,19 @@ def cpu_baseline(wl, sectors: int, threads: int, keep_surfaces: bool = False):
                        f"restatement of the reference CPU loop, {threads} thread(s), "
                        f"{dt:.1f} s on {cpu}; band data (axes sorts + compile) timed "
                        f"separately, {threads} thread(s)")}
+    if fp64:
+        t0 = time.perf_counter()
+        f64 = O.walk_sectors_f64(dem, elev, wl.observer_height, deltas, dists,
+                                 n_threads=threads)
+        res["fp64_walk_s"] = time.perf_counter() - t0
+        return res, surfaces, f64
     return (res, surfaces) if keep_surfaces else res
+
+
+def rel_err_stats(got: np.ndarray, ref: np.ndarray) -> dict:
+    err = np.abs(got.astype(np.float64) - ref) / np.maximum(np.abs(ref), 1e-300)
+    return {"max": float(np.max(err)), "p99": float(np.percentile(err, 99)),
+            "median": float(np.median(err))}
 
 
 def parse_shape(text: str) -> tuple:
@@ -714,12 +730,25 @@ def main() -> int:
         # single-thread sample.
         cores = min(16, len(os.sched_getaffinity(0)))
         if cores > 1:
-            out["cpu_all_cores"], cpu_surf = cpu_baseline(wl, 180, cores, keep_surfaces=True)
+            out["cpu_all_cores"], cpu_surf, (tvs64, flips, steps64) = cpu_baseline(
+                wl, 180, cores, keep_surfaces=True, fp64=True)
             # The oracle's 180-sector TVS against the batched pass checked above.
             out["tvs_bit_exact_vs_oracle"] = bool(
                 np.array_equal(cpu_surf.view(np.uint32), tvs_b.view(np.uint32)))
+            # The precision form against the FP64 oracle (the same band data,
+            # angles, maxima and sums in f64): the f32 walk's visibility
+            # flips, and the relative error of the batched pass's f32 TVS and
+            # of its f64 per-point sums.
+            out["fp64_oracle"] = {
+                "point_steps": steps64, "visibility_flips": flips,
+                "flip_rate": flips / steps64 if steps64 else None,
+                "tvs_f32_rel_err": rel_err_stats(tvs_b, tvs64),
+                "tvs_f64_sum_rel_err": rel_err_stats(tvs_b64, tvs64),
+                "cpu_walk_s": out["cpu_all_cores"].pop("fp64_walk_s")}
             log(f"CPU, {cores} threads: {out['cpu_all_cores']['value']:.3e} point-steps/s; "
-                f"batched TVS bit-exact vs the oracle: {out['tvs_bit_exact_vs_oracle']}")
+                f"batched TVS bit-exact vs the oracle: {out['tvs_bit_exact_vs_oracle']}; "
+                f"FP64 oracle: {flips} flips, f32 TVS rel err p99 "
+                f"{out['fp64_oracle']['tvs_f32_rel_err']['p99']:.2e}")
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
