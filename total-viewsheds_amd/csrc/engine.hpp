@@ -31,11 +31,14 @@ constexpr uint32_t kBandSlots = kSectorSteps + 1;
 // dword in copy A and fill one aligned dword in copy B; the step tables
 // send each step whose two-point loads would be misaligned in A to B.
 // Both the int16 and the f32 buffers have the layout, so the tables do not
-// depend on the uploaded height type.  Larger DEMs have no copy: there the
-// doubled footprint cost more cache hits than the aligned loads saved
-// (measured: 1201^2 and the 64 tiles 4 % faster with the copy, 3601^2 3 %
-// slower).
-constexpr uint64_t kDemCopyMaxPoints = uint64_t(1) << 22;
+// depend on the uploaded height type.  Measured: 1201^2 and the 64 tiles
+// 4 % faster with the copy; 3601^2 3 % slower while every step ran the
+// masked update, 4 % faster in the walk's directional item order once the
+// update is skipped where no lane sees the point (34.0 against 35.5 ms;
+// the XCD-contiguous order 2 % slower, so the tuner picks).  The limit
+// keeps every step's byte position, copy included, below 2^32 with room
+// (at most ~4 n int16 bytes, twice that for f32 heights).
+constexpr uint64_t kDemCopyMaxPoints = uint64_t(1) << 24;
 inline uint64_t dem_copy_offset(uint64_t n) { return n <= kDemCopyMaxPoints ? (n | 1u) : 0u; }
 inline uint64_t dem_span(uint64_t n) { return dem_copy_offset(n) + n; }
 // Parity of the DEM index of every two-point pair's first point: pairs
