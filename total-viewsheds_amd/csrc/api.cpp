@@ -822,11 +822,19 @@ tvs_status run_sectors(tvs_ctx *c, uint32_t first, uint32_t count, tvs_method me
         if ((st = tvs::ensure_short_div(c, c->tab, first, count)) != TVS_OK) return st;
         const uint32_t n_items = d.blocks_per_pass * 2u * count;
         const uint32_t begin[4] = {0u, n_items, 0u, N};
-        TVS_HIP(c, hipMemcpyAsync(c->d_tile, &d, sizeof d, hipMemcpyHostToDevice, c->stream),
-                "tile descriptor");
-        TVS_HIP(c, hipMemcpyAsync(c->d_begin, begin, sizeof begin, hipMemcpyHostToDevice,
-                                  c->stream),
-                "tile descriptor");
+        if (!c->desc_cached || std::memcmp(&d, &c->desc_on_device, sizeof d) != 0 ||
+            std::memcmp(begin, c->begin_on_device, sizeof begin) != 0) {
+            c->desc_cached = false;
+            TVS_HIP(c, hipMemcpyAsync(c->d_tile, &d, sizeof d, hipMemcpyHostToDevice,
+                                      c->stream),
+                    "tile descriptor");
+            TVS_HIP(c, hipMemcpyAsync(c->d_begin, begin, sizeof begin, hipMemcpyHostToDevice,
+                                      c->stream),
+                    "tile descriptor");
+            c->desc_on_device = d;
+            std::memcpy(c->begin_on_device, begin, sizeof begin);
+            c->desc_cached = true;
+        }
         TVS_HIP(c, hipEventRecord(c->ev[0], c->stream), "event");
         TVS_HIP(c, hipEventRecord(hist[0], c->stream), "event");
         TVS_HIP(c, tvs::launch_walk_tiles(c->stream, c->d_tile, c->d_begin, 1, n_items,
@@ -908,6 +916,7 @@ tvs_status tvs_compact_rings(tvs_ctx *c, uint32_t sector, uint32_t *counts, size
     const uint32_t begin[4] = {0u, n_items, 0u, N};
     d.ring_mode = tvs::kRingCount;
     d.ring_counts = c->d_ring_counts;
+    c->desc_cached = false;
     TVS_HIP(c, hipMemcpyAsync(c->d_begin, begin, sizeof begin, hipMemcpyHostToDevice, c->stream), "descriptor");
     // Counts and offsets of this sector are kept until the next upload or
     // band change, so the usual pair of calls (size query, then events)

@@ -224,6 +224,13 @@ struct tvs_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {};
+    // What the last batched run copied into d_tile / d_begin (valid while
+    // desc_cached): an identical descriptor is not copied again, which
+    // spares back-to-back runs two host-to-device copies each.  Every other
+    // writer of d_tile / d_begin clears desc_cached.
+    tvs::TileDesc desc_on_device{};
+    uint32_t begin_on_device[4] = {};
+    bool desc_cached = false;
     // Start and end of the walk of each of the last kWalkHistory runs of
     // tvs_run_sectors (slot = run index mod kWalkHistory), created on the
     // first run.

@@ -89,6 +89,7 @@ tvs_status run_chunks(tvs_ctx *c, uint32_t first, uint32_t count) {
     tvs::TileDesc descs[tvs::kSectorSteps];
     uint32_t begins[tvs::kSectorSteps][4];
     uint32_t launches = 0;
+    c->desc_cached = false;   // the chunks' descriptors go through d_tile / d_begin
     auto enqueue = [&]() -> tvs_status {
         TVS_NHIP(c, hipEventRecord(c->ev[0], c->stream), "event");
         for (uint32_t s = first, j = 0; s < first + count; s += per_chunk, ++j) {
