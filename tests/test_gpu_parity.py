@@ -114,6 +114,35 @@ def test_full_int16_range_bit_exact(oracle):
                                           err_msg=np.dtype(dtype).name)
 
 
+def test_tile_descriptor_reuse_across_entry_points(small):
+    """Back-to-back batched runs reuse the tile descriptor already on the
+    device; the ring export and the no-allocation run write the same device
+    descriptor buffers, so a batched run after either must copy its own
+    again.  Every run here starts from zero surfaces and must equal the
+    oracle's 180-sector TVS."""
+    wl, _, _, _, cum = small
+    with make_engine(wl) as e:
+        e.upload(wl.elevations)
+        e.compute_bands(0, 180)
+        for step in ["run", "run", "rings", "run", "noalloc", "run", "rings", "noalloc", "run"]:
+            note(f"descriptor reuse: {step}")
+            if step == "rings":
+                e.compact_rings(17)
+                continue
+            e.reset_surfaces()
+            if step == "run":
+                e.run(0, 180, "batched")
+            else:
+                e.run_noalloc(0, 180)
+            np.testing.assert_array_equal(e.surfaces().view(np.uint32), cum.view(np.uint32),
+                                          err_msg=step)
+        # A different sector range, then the full range again.
+        e.reset_surfaces()
+        e.run(0, 90, "batched")
+        e.run(90, 90, "batched")
+        np.testing.assert_array_equal(e.surfaces().view(np.uint32), cum.view(np.uint32))
+
+
 @pytest.mark.parametrize("terrain", ["flat", "spikes"])
 def test_update_branch_extremes(oracle, terrain):
     """The batched walk skips a band's masked update when no lane of the
@@ -490,6 +519,31 @@ def test_main_full_run_methods_agree(main):
     e.run(0, 180, "batched")
     np.testing.assert_array_equal(e.surfaces().view(np.uint32), plain.view(np.uint32))
     e.set_short_divide(True)
+
+
+def test_main_descriptor_reuse_after_noalloc_chunks(main):
+    """At 1201 x 1201 the no-allocation run walks the 180 sectors in four
+    chunks, the last one 24 sectors wide, through the same device descriptor
+    buffers as the batched run: a batched run right after it must copy its
+    own descriptor again (a stale one would walk the last chunk only).  The
+    same after a ring export of one sector.  Compared with a batched run
+    from a fresh descriptor state (the plain walk equals it, test above)."""
+    wl, dem, e = main
+    e.reset_surfaces()
+    e.run(0, 180, "batched")
+    want = e.surfaces()
+    for step in ["noalloc", "run", "rings", "run", "run"]:
+        note(f"main-1201 descriptor reuse: {step}")
+        if step == "rings":
+            e.compact_rings(91)
+            continue
+        e.reset_surfaces()
+        if step == "run":
+            e.run(0, 180, "batched")
+        else:
+            e.run_noalloc(0, 180)
+        np.testing.assert_array_equal(e.surfaces().view(np.uint32), want.view(np.uint32),
+                                      err_msg=step)
 
 
 def test_main_rings_one_sector(main, oracle):
