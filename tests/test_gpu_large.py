@@ -2,7 +2,8 @@
 the C ABI.  The oracle checks band data of all 180 sectors, whole sectors
 walked alone in three launch shapes, and the cumulative surfaces after
 every 30 sectors up to the full 180-sector pass; the full run is also
-checked as batched = plain."""
+checked as batched = plain; two sectors' ring events against the
+oracle's records."""
 import os
 import sys
 
@@ -144,3 +145,31 @@ def test_large_all_sectors_equal_oracle(large, large_all_bands, oracle, end):
     want = _oracle_prefix(oracle, wl, dem, large_all_bands, end)
     np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32),
                                   err_msg=f"sectors 0..{end - 1}")
+
+
+@pytest.mark.parametrize("sector", [100, 179])
+def test_large_ring_events_equal_oracle(large, large_all_bands, oracle, sector):
+    """One sector's ring events at 3601 x 3601 (1 740 978 bands of 1333
+    steps; sector 179 opens with a step at distance +0): the compact export
+    equals the oracle's fixed-slot records (kernel.rs:141-243) band for band
+    -- count slot, PoV slot, and the alternating close/open DEM ids.  No band
+    fills its R slots here, so the oracle's records hold every event."""
+    wl, dem, e = large
+    R = oracle.ring_count_per_band(wl.rings_per_km, wl.max_line_of_sight)
+    note(f"large-3601 ring events sector {sector}: engine export")
+    counts, events = e.compact_rings(sector)
+    assert counts.size == 2 * dem.computable_points_count
+    assert int(counts.max()) < R, "a band overflows its R slots"
+    deltas, dists = large_all_bands
+    note(f"large-3601 ring events sector {sector}: oracle records")
+    _, want = oracle.walk_sectors(dem, wl.elevations.astype(np.float32), wl.observer_height,
+                                  deltas[sector:sector + 1], dists[sector:sector + 1],
+                                  rings_per_band=R, rings=True, n_threads=THREADS)
+    want = want[0].reshape(-1, R)
+    np.testing.assert_array_equal(want[:, 0], counts)
+    from tvs_amd import rings
+    np.testing.assert_array_equal(want[:, 1], rings.band_povs(e.geometry))
+    # Slots 2 .. count of every band, row after row, are the packed events.
+    slots = np.arange(2, R)[None, :] <= (want[:, :1].astype(np.int64))
+    np.testing.assert_array_equal(want[:, 2:][slots], events)
+    assert not np.any(want[:, 2:][~slots])
