@@ -2,9 +2,11 @@
 
 Bit-exact: band data (sort order, deltas, distances), full Axes orders,
 cumulative f32 surfaces of both walk methods, ring records.  At the
-1201 x 1201 size the oracle runs on a sample of sectors; the full 180-sector
-run is checked by size-independent properties (the two methods agree bit
-for bit; the f64 sums agree with the f32 ones within 1e-5 relative)."""
+1201 x 1201 size the oracle checks band data and the ring events of all 180
+sectors and the surfaces of a sample of sectors (test_gpu_configs.py holds
+the full 180-sector TVS); the full run is also checked by size-independent
+properties (the two methods agree bit for bit; the f64 sums agree with the
+f32 ones within 1e-5 relative)."""
 import os
 import sys
 
@@ -559,6 +561,32 @@ def test_main_rings_one_sector(main, oracle):
         e.set_band(s, d, x)
         e.run(s, 1, "batched")
         np.testing.assert_array_equal(e.rings(0), want[0])
+
+
+def test_main_ring_events_all_sectors(main, oracle):
+    """All 180 sectors of main-1201 (321 602 bands each): the compact ring
+    export against the oracle's records, band for band (count slot, PoV
+    slot, the alternating close/open DEM ids; kernel.rs:141-243).  No band
+    fills its R = 180 slots."""
+    wl, dem, e = main
+    from tvs_amd import rings
+    R = oracle.ring_count_per_band(wl.rings_per_km, wl.max_line_of_sight)
+    povs = rings.band_povs(e.geometry)
+    elev = wl.elevations.astype(np.float32)
+    for s in range(180):
+        if s % 30 == 0:
+            note(f"main-1201 ring events sectors {s}..{s + 29}")
+        counts, events = e.compact_rings(s)
+        assert int(counts.max()) < R
+        d, x = oracle.band_data(dem, s)
+        _, want = oracle.walk_sectors(dem, elev, wl.observer_height, d[None], x[None],
+                                      rings_per_band=R, rings=True, n_threads=THREADS)
+        want = want[0].reshape(-1, R)
+        np.testing.assert_array_equal(want[:, 0], counts, err_msg=f"sector {s}")
+        np.testing.assert_array_equal(want[:, 1], povs)
+        slots = np.arange(2, R)[None, :] <= want[:, :1].astype(np.int64)
+        np.testing.assert_array_equal(want[:, 2:][slots], events, err_msg=f"sector {s}")
+        assert not np.any(want[:, 2:][~slots])
 
 
 # --- error paths through the ABI --------------------------------------------
