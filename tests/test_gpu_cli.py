@@ -56,6 +56,40 @@ def test_cli_heatmap(tmp_path, small, method):
                                   png_of(cum, dem.tvs_width))
 
 
+def test_cli_tune_caches_the_walk_shape(tmp_path, small):
+    """--tune times the walk's launch shapes on the DEM and keeps the
+    fastest; the choice is written under the state directory per width,
+    line of sight and height type, and the next run loads it instead of
+    tuning again.  Both runs draw the oracle's picture."""
+    wl, dem, cum = small
+    path = tmp_path / "dem.bt"
+    bt.write(str(path), wl.elevations.reshape(wl.width, wl.width))
+    state = tmp_path / "state"
+    out = tmp_path / "out"
+    args = [CLI, "--input", str(path), "--scale", "100", "--max-line-of-sight", "6700",
+            "--output-dir", str(out), "--state-dir", str(state), "--tune"]
+    print(" ".join(args), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "Tuning the walk's launch shape" in r.stderr
+    shape_file = state / "total-viewsheds" / "201" / "walk_shape" / "6700_i16.txt"
+    fields = shape_file.read_text().split()
+    assert len(fields) == 7 and int(fields[0]) in (64, 128, 256, 512)
+    np.testing.assert_array_equal(read_png_grey(str(out / "heatmap.png")),
+                                  png_of(cum, dem.tvs_width))
+    r2 = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r2.returncode == 0, r2.stderr
+    assert "Tuning" not in r2.stderr and "Walk shape from" in r2.stderr
+    np.testing.assert_array_equal(read_png_grey(str(out / "heatmap.png")),
+                                  png_of(cum, dem.tvs_width))
+    # A stored shape this build does not have is ignored: tuned again.
+    shape_file.write_text("48 3 0 0 0 0 0\n")
+    r3 = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r3.returncode == 0, r3.stderr
+    assert "Tuning the walk's launch shape" in r3.stderr
+    assert shape_file.read_text().split()[0] in ("64", "128", "256", "512")
+
+
 def test_cli_default_line_of_sight_and_errors(tmp_path, small):
     wl, dem, _ = small
     path = tmp_path / "dem.bt"

@@ -27,6 +27,7 @@ struct Config {
     std::string state_dir;              // band-data cache root
     bool use_cache = true;
     bool heatmap_each_sector = false;
+    bool tune = false;                  // time the walk shapes before the run
     bool verbose = false;
 };
 
@@ -48,9 +49,13 @@ void usage(std::FILE *f) {
         "      --scale <metres>              Override the DEM point spacing from the file\n"
         "      --state-dir <dir>             Band-data cache root [default: $XDG_STATE_HOME or\n"
         "                                    ~/.local/state]\n"
-        "      --no-cache                    Always compute band data on the device\n"
+        "      --no-cache                    Always compute band data (and, with --tune, the walk\n"
+        "                                    shape) on the device\n"
         "      --heatmap-each-sector         Rewrite heatmap.png after every sector with the running\n"
         "                                    total, as the reference does\n"
+        "      --tune                        Time the walk's launch shapes on this DEM first and\n"
+        "                                    keep the fastest; the choice is cached per width, line\n"
+        "                                    of sight and height type in the state directory\n"
         "  -v, --verbose                     Per-sector log lines\n"
         "  -h, --help                        Print help\n");
 }
@@ -77,6 +82,7 @@ bool parse(int argc, char **argv, Config &c) {
         else if (a == "-v" || a == "--verbose") c.verbose = true;
         else if (a == "--no-cache") c.use_cache = false;
         else if (a == "--heatmap-each-sector") c.heatmap_each_sector = true;
+        else if (a == "--tune") c.tune = true;
         else if (a == "--input") { if (!need("--input")) return false; c.input = v; }
         else if (a == "--max-line-of-sight") { if (!need(a.c_str())) return false; c.max_line_of_sight = std::atoll(v.c_str()); }
         else if (a == "--rings-per-km") { if (!need(a.c_str())) return false; c.rings_per_km = std::strtof(v.c_str(), nullptr); }
@@ -107,6 +113,46 @@ int die(tvs_ctx *ctx, tvs_status st, const char *what) {
                  ctx ? tvs_last_error(ctx) : tvs_last_error(nullptr), tvs_status_string(st));
     if (ctx) tvs_destroy(ctx);
     return 1;
+}
+
+// mkdir -p.
+bool make_dirs(const std::string &path) {
+    for (size_t i = 1; i <= path.size(); ++i) {
+        if (i < path.size() && path[i] != '/') continue;
+        const std::string part = path.substr(0, i);
+        if (::mkdir(part.c_str(), 0755) != 0 && errno != EEXIST) return false;
+    }
+    return true;
+}
+
+// The walk shape chosen by --tune for one geometry and height type, under
+// <state>/total-viewsheds/<width>/walk_shape/<line of sight>_<i16|f32>.txt
+// as the seven integers of tvs_walk_shape (block, unroll, xcd_contiguous,
+// pipeline, persistent, both_directions, two_points).
+std::string shape_path(const std::string &state, uint32_t width, uint32_t los, bool is_float) {
+    return state + "/total-viewsheds/" + std::to_string(width) + "/walk_shape/" +
+           std::to_string(los) + (is_float ? "_f32" : "_i16") + ".txt";
+}
+
+bool load_shape(const std::string &path, tvs_walk_shape *out) {
+    std::FILE *f = std::fopen(path.c_str(), "r");
+    if (!f) return false;
+    tvs_walk_shape w{};
+    const int n = std::fscanf(f, "%u %u %u %u %u %u %u", &w.block, &w.unroll, &w.xcd_contiguous,
+                              &w.pipeline, &w.persistent, &w.both_directions, &w.two_points);
+    std::fclose(f);
+    if (n != 7) return false;
+    *out = w;
+    return true;
+}
+
+bool save_shape(const std::string &path, const tvs_walk_shape &w) {
+    if (!make_dirs(path.substr(0, path.rfind('/')))) return false;
+    std::FILE *f = std::fopen(path.c_str(), "w");
+    if (!f) return false;
+    std::fprintf(f, "%u %u %u %u %u %u %u\n", w.block, w.unroll, w.xcd_contiguous, w.pipeline,
+                 w.persistent, w.both_directions, w.two_points);
+    return std::fclose(f) == 0;
 }
 
 std::string default_state_dir() {
@@ -198,6 +244,24 @@ int main(int argc, char **argv) {
     }
 
     const tvs_method method = cfg.compute == "hip" ? TVS_METHOD_BATCHED : TVS_METHOD_PLAIN;
+    if (cfg.tune && method == TVS_METHOD_BATCHED) {
+        // A cached choice for this geometry and height type, else the tuner
+        // (it keeps the fastest shape as the context's; surfaces untouched).
+        const std::string sp = state.empty() ? "" : shape_path(state, hdr.width, los, hdr.is_float);
+        tvs_walk_shape best{};
+        if (cfg.use_cache && !sp.empty() && load_shape(sp, &best) &&
+            tvs_set_walk_shape(ctx, &best) == TVS_OK) {
+            std::fprintf(stderr, "Walk shape from %s: %u x %u\n", sp.c_str(), best.block, best.unroll);
+        } else {
+            std::fprintf(stderr, "Tuning the walk's launch shape on this DEM\n");
+            if ((st = tvs_autotune(ctx, 0, 180, 2, nullptr, 0, nullptr, &best)) != TVS_OK)
+                return die(ctx, st, "tuning");
+            std::fprintf(stderr, "Walk shape: %u x %u, %.2f ms per pass\n", best.block, best.unroll,
+                         best.ms);
+            if (cfg.use_cache && !sp.empty() && !save_shape(sp, best))
+                std::fprintf(stderr, "warning: walk shape not saved to %s\n", sp.c_str());
+        }
+    }
     std::vector<float> surfaces(g.computable_points_count);
     const std::string png = cfg.output_dir + "/heatmap.png";
     ::mkdir(cfg.output_dir.c_str(), 0755);
