@@ -44,14 +44,20 @@ SIMDS = 256 * 4
 CLOCK_HZ = 2.4e9
 VALU_ISSUE_PEAK_LANE_OPS = SIMDS * (2.0 / 4.0) * CLOCK_HZ * 64
 # Vector instructions every real band-step of the batched walk's unrolled
-# loop executes: sub (height difference), the divide (4 operations, or 2
-# with the verified short divide), sub (curvature), compare, plus one
-# conversion when two points per lane share a dword load
-# (tests/test_codegen.py holds the compiled loops to these counts).  The
-# masked surface add and maximum move run only where a lane of the wave sees
-# the point (~2 % of a lane's band-steps, ~35 % of a wave's at 1201 x 1201)
-# and are not counted, so `achieved` is the work the walk cannot skip.
+# loop needs: the height difference, the divide (4 operations, or 2 with
+# the verified short divide), sub (curvature), compare -- plus one
+# conversion where two points per lane share an int16 dword load and the
+# DEM has no f16 form (with it, one mixed-precision FMA converts the f16
+# height and subtracts the observer's; one point per lane loads with a typed
+# load that converts) (tests/test_codegen.py holds the compiled loops to
+# these counts).  The masked surface add and maximum move run only where a
+# lane of the wave sees the point (~2 % of a lane's band-steps, ~35 % of a
+# wave's at 1201 x 1201) and are not counted, so `achieved` is the work the
+# walk cannot skip.
 VALU_PER_STEP = {"split": 7, "short": 5}
+
+# --no-half-heights: walk int16 DEMs with the int16 decode only.
+HALF_HEIGHTS = True
 
 
 def log(msg: str) -> None:
@@ -157,6 +163,7 @@ def measure_large(steps: int, warmup: int, cpu: bool = False, shape_arg=None) ->
                         wl.observer_height, 0) as e:
         g = e.geometry
         e.upload(wl.elevations)
+        e.set_half_heights(HALF_HEIGHTS)
         e.compute_bands(0, 180)
         bands_ms = e.timing().bands_ms
         if shape_arg:
@@ -226,6 +233,7 @@ def measure_tiles(steps: int, warmup: int, cpu: bool = False, shape_arg=None) ->
                          t.observer_height) for t in tiles]) as b:
         for i, t in enumerate(tiles):
             b.upload(i, t.elevations)
+        b.set_half_heights(HALF_HEIGHTS)
         b.compute_bands()
         bands_ms = b.timing().bands_ms
         if shape_arg:
@@ -486,6 +494,8 @@ def main() -> int:
                          "lane (0, 1, or 2 = two pairs from adjacent rows) "
                          "(e.g. 64,8,1,0,0,1,1)")
     ap.add_argument("--no-f32", action="store_true", help="skip the f32-heights batched run")
+    ap.add_argument("--no-half-heights", action="store_true",
+                    help="walk int16 DEMs without their f16 form (int16 decode only)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the large-3601 and tiles-64 figures")
     ap.add_argument("--cpu-sectors", type=int, default=36)
@@ -506,6 +516,8 @@ def main() -> int:
                              "COPY_BYTES", "OUT_JSON"),
                     help="turn counter CSVs into HBM bytes per walk launch")
     args = ap.parse_args()
+    global HALF_HEIGHTS
+    HALF_HEIGHTS = not args.no_half_heights
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
     if args.band_paths:
@@ -532,6 +544,7 @@ def main() -> int:
                        wl.observer_height, 0)
     g = e.geometry
     e.upload(wl.elevations)
+    e.set_half_heights(HALF_HEIGHTS)
     t0 = time.perf_counter()
     e.compute_bands(0, 180)
     bands_wall_ms = (time.perf_counter() - t0) * 1000.0
@@ -615,6 +628,7 @@ def main() -> int:
     e.reset_surfaces()
     e.run(0, 180, "plain")
     tvs_p = e.surfaces()
+    half_heights = all(e.half_heights())
     e.close()
     rel_err = float(np.max(np.abs(tvs_b.astype(np.float64) - tvs_p) / tvs_p))
     rel_err64 = float(np.max(np.abs(tvs_b64 - tvs_p) / tvs_b64))
@@ -622,8 +636,8 @@ def main() -> int:
 
     gather_bytes = real_point_steps * 2     # one int16 height per executed step, L2-served
     per_step = VALU_PER_STEP["short"] if short_sectors == 180 else VALU_PER_STEP["split"]
-    if shape["two_points"]:
-        per_step += 1   # the two heights of a dword are converted by one instruction each
+    if shape["two_points"] and not half_heights:
+        per_step += 1   # the two int16 heights of a dword are converted by one instruction each
     lane_ops = real_point_steps * per_step          # VALU lane-operations per launch
     valu_lane_ops = lane_ops / (b_kernel_ms / 1000.0)
     traffic = None
@@ -694,6 +708,7 @@ def main() -> int:
             "kernel": "walk_tiles_kernel<int16_t> (batched walk)",
             "kernel_ms": b_kernel_ms,
             "valu_per_band_step": per_step,
+            "half_heights": half_heights,
             "short_divide_sectors": short_sectors,
             "lane_ops_per_launch": lane_ops,
             "peak_basis": "1024 SIMDs x 2 wave64 VALU instructions per 4 clocks (dual issue) "

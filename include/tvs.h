@@ -156,6 +156,18 @@ tvs_status tvs_compute_bands(tvs_ctx *ctx, uint32_t first_sector,
 tvs_status tvs_set_short_divide(tvs_ctx *ctx, uint32_t enable);
 tvs_status tvs_get_short_divide(tvs_ctx *ctx, uint32_t *enabled,
                                 uint32_t *verified_sectors);
+/* The batched walk's height decode.  With half heights on (the default),
+ * an int16 DEM whose heights span at most 4096 is also kept as f16 values
+ * h - C (C an integer, every |h - C| <= 2048, all exact); the two-point
+ * walk then forms each height difference h - h0 in one mixed-precision
+ * FMA, h16 + (C - h0), for every wave whose observers all have C - h0
+ * exact in f32 (it checks), instead of converting the int16 height and
+ * subtracting.  The difference is the same f32 value either way, so results
+ * are identical.  `available` = 1 when the uploaded DEM has the f16 form.
+ * No reference counterpart (kernel.rs:160-175 subtracts f32 heights). */
+tvs_status tvs_set_half_heights(tvs_ctx *ctx, uint32_t enable);
+tvs_status tvs_get_half_heights(const tvs_ctx *ctx, uint32_t *enabled,
+                                uint32_t *available);
 /* `holds` = 1 when the uploaded heights admit the four-operation divide:
  * integer heights whose differences from every observer height stay within
  * the proof's range, or non-integer heights that are all 0 or of magnitude
@@ -374,6 +386,8 @@ tvs_status tvs_batch_compute_bands(tvs_batch *batch);
 tvs_status tvs_batch_set_walk_shape(tvs_batch *batch, const tvs_walk_shape *shape);
 /* tvs_set_short_divide for every tile. */
 tvs_status tvs_batch_set_short_divide(tvs_batch *batch, uint32_t enable);
+/* tvs_set_half_heights for every tile. */
+tvs_status tvs_batch_set_half_heights(tvs_batch *batch, uint32_t enable);
 /* tvs_autotune for a batch: times the walk launch over all tiles in every
  * compiled shape and keeps the fastest.  Only the walk runs, so the tiles'
  * cumulative surfaces are unchanged.  Needs band data and elevations. */

@@ -152,12 +152,39 @@ def test_eight_band_walk_loop_shape(asm):
     assert scalar_operand_valu(short) <= 16 and scalar_operand_valu(split) <= 20
     four = unrolled_loops(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb1ELi1ELb0E"))
     four_short = min((l for l in four if "v_div_scale" not in l and
-                      "buffer_load_format_x" not in l), key=valu)
+                      "buffer_load_format_x" not in l and "v_fma_mix_f32" not in l), key=valu)
     assert valu(four_short) <= 8 * 32 + 8 + 8
     assert scalar_operand_valu(four_short) >= valu(four_short) // 3
     # The four-band loop copies the curvature term into a vector register
     # once per step (8 per chunk), nothing else.
     assert len(re.findall(r"v_mov_b32(?:_e32)?\s+v\d+, s\d+", four_short)) == 8
+
+
+def test_half_height_loop_shape(asm):
+    """int16 heights with their f16 form, two points and both directions per
+    lane: the height difference of every band-step is one v_fma_mix_f32 from
+    the loaded dword (no conversion, no separate subtraction), 16 dword loads
+    per 8-step chunk; 7.5 vector instructions per band-step with the short
+    divide (the masked update's two included), three quarters of an
+    instruction fewer than the int16 loop of the same kernel."""
+    loops = unrolled_loops(kernel_body(asm, "IsLb0ELi64ELi8ELb0ELb0ELb1ELi1ELb0E"))
+    fast = [l for l in loops if "v_div_scale" not in l and "buffer_load_format_x" not in l]
+    half = [l for l in fast if "v_fma_mix_f32" in l]
+    cvt = [l for l in fast if "v_fma_mix_f32" not in l]
+    assert len(half) == 2 and len(cvt) == 2
+    for loop in half:
+        check_updates(loop, 32)
+        assert len(re.findall(r"v_fma_mix_f32\s", loop)) == 32
+        assert "v_cvt_f32_i32" not in loop
+        assert len(re.findall(r"buffer_load_dword\s+v\d+, v\d+, s\[\d+:\d+\], s\d+ offen",
+                              loop)) == 16
+    # 32 conversions and 32 subtractions become 32 mixed FMAs; the step's
+    # curvature term and reciprocal high part are copied into vector
+    # registers (16 copies per chunk, 8 in the int16 loop).
+    assert min(valu(l) for l in half) + 32 - 8 == min(valu(l) for l in cvt)
+    assert min(valu(l) for l in half) <= 7 * 32 + 16 + 8
+    for loop in half:
+        assert len(re.findall(r"v_mov_b32(?:_e32)?\s+v\d+, s\d+", loop)) == 16
 
 
 def test_eight_band_kernel_holds_five_waves(asm):

@@ -117,6 +117,27 @@ def test_batch_shapes_agree(tiles_and_truth):
                 np.testing.assert_array_equal(b.surfaces(i), truth[i])
 
 
+def test_batch_half_heights_toggle(tiles_and_truth):
+    """The small tiles' heights (0 to 4000 m) all have the f16 form; the
+    batch's two-point walk with it on, off and on again (the graph is
+    rebuilt each time) equals the oracle bit for bit."""
+    tiles, truth = tiles_and_truth
+    with tvs_amd.Batch([(t.width, t.spacing, t.max_line_of_sight, t.rings_per_km,
+                         t.observer_height) for t in tiles]) as b:
+        for i, t in enumerate(tiles):
+            b.upload(i, t.elevations)
+        b.compute_bands()
+        b.set_walk_shape(64, 8, True, False, both_directions=True, two_points=1)
+        for on in (True, False, True):
+            print(f"batch half heights {on}", flush=True)
+            b.set_half_heights(on)
+            b.reset_surfaces()
+            b.run(True)
+            for i in range(len(tiles)):
+                np.testing.assert_array_equal(b.surfaces(i).view(np.uint32),
+                                              truth[i].view(np.uint32), err_msg=f"{on} tile {i}")
+
+
 def test_autotune_keeps_results_exact(tiles_and_truth):
     tiles, truth = tiles_and_truth
     t = tiles[0]

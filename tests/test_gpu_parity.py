@@ -145,6 +145,42 @@ def test_tile_descriptor_reuse_across_entry_points(small):
         np.testing.assert_array_equal(e.surfaces().view(np.uint32), cum.view(np.uint32))
 
 
+@pytest.mark.parametrize("lo,hi", [(0, 1500), (0, 4000), (-1000, 3096), (100, 4196),
+                                   (0, 5000)])
+def test_half_heights_bit_exact(oracle, lo, hi):
+    """int16 DEMs whose heights span at most 4096 also get the f16 form
+    h - C; the two-point walk forms h - h0 from it in one mixed FMA in every
+    wave whose observers all have an exact C - h0, and converts the int16
+    heights in the others.  Ranges: C = 0 (every wave); C = 1952 and 2196
+    (the low observers' waves fall back); a negative range around C = 0; a
+    span of 4097 (no f16 form).  With the form on and off the TVS equals the
+    oracle's bit for bit."""
+    w, spacing, los = 201, 90.0, 6000
+    elev = (demgen.fractal_dem(w, spacing, 2.4, float(hi - lo), seed=lo + hi).astype(np.int64)
+            + lo)
+    elev[0, 0], elev[-1, -1] = lo, hi
+    elev = elev.astype(np.int16).reshape(-1)
+    note(f"half heights [{lo}, {hi}]: oracle band data + walk")
+    dem = oracle.dem_new(w, spacing, los)
+    deltas, dists = oracle.band_data_all(dem, range(180))
+    cum, _ = oracle.walk_sectors(dem, elev.astype(np.float32), 1.65, deltas, dists,
+                                 n_threads=THREADS)
+    with tvs_amd.Engine(w, spacing, los, 5.0, 1.65, 0) as e:
+        e.upload(elev)
+        assert e.half_heights() == (True, hi - lo <= 4096)
+        e.compute_bands(0, 180)
+        for on in (True, False):
+            e.set_half_heights(on)
+            for shape in [(64, 8, True, False, False, True, 1),
+                          (128, 4, True, False, False, True, 1)]:
+                note(f"half heights [{lo}, {hi}]: on={on}, shape {shape}")
+                e.set_walk_shape(*shape)
+                e.reset_surfaces()
+                e.run(0, 180, "batched")
+                np.testing.assert_array_equal(e.surfaces().view(np.uint32),
+                                              cum.view(np.uint32), err_msg=f"{on} {shape}")
+
+
 @pytest.mark.parametrize("terrain", ["flat", "spikes"])
 def test_update_branch_extremes(oracle, terrain):
     """The batched walk skips a band's masked update when no lane of the

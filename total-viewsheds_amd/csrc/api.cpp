@@ -320,6 +320,8 @@ tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t cap
                         : static_cast<const void *>(c->elev_f32);
     d.span = static_cast<uint32_t>(dem_span(w2));
     d.copy_offset = static_cast<uint32_t>(dem_copy_offset(w2));
+    d.elev_h16 = c->has_i16 && c->has_h16 && c->half_heights ? c->elev_h16 : nullptr;
+    d.h16_offset = c->h16_offset;
     d.off = c->tab.off;
     d.pos_f = c->tab.pos_f;
     d.pos_b = c->tab.pos_b;
@@ -479,6 +481,7 @@ tvs_status tvs_destroy(tvs_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     dev_free(c->elev_f32);
     dev_free(c->elev_i16);
+    dev_free(c->elev_h16);
     dev_free(c->tab.delta);
     dev_free(c->tab.dist);
     dev_free(c->tab.off);
@@ -551,6 +554,7 @@ tvs_status tvs_upload_dem_f32(tvs_ctx *c, const float *elev, size_t count) {
     TVS_HIP(c, hipStreamSynchronize(c->stream), "elevation upload");
     dev_free(c->elev_i16);
     c->has_i16 = false;
+    c->has_h16 = false;
     c->has_f32 = true;
     bool integral = true;
     double lo = INFINITY, hi = -INFINITY;
@@ -592,6 +596,21 @@ tvs_status tvs_upload_dem_i16(tvs_ctx *c, const int16_t *elev, size_t count) {
     for (size_t i = 0; i < count; ++i) {
         lo = std::min<int64_t>(lo, elev[i]);
         hi = std::max<int64_t>(hi, elev[i]);
+    }
+    // Heights within 4096 of each other also as f16 offsets from an integer
+    // C with every |h - C| <= 2048 (f16 holds those integers exactly): C is
+    // the value of [hi - 2048, lo + 2048] nearest 0, which keeps C - h0 exact
+    // in f32 for the most observers (the walk checks each one).
+    c->has_h16 = false;
+    if (hi - lo <= 4096) {
+        const int64_t off = std::min(std::max<int64_t>(0, hi - 2048), lo + 2048);
+        if (!c->elev_h16) TVS_HIP(c, dev_alloc(&c->elev_h16, tvs::dem_span(count)), "elevations");
+        TVS_HIP(c, tvs::launch_half_heights(c->stream, c->elev_i16, c->elev_h16,
+                                            tvs::dem_span(count), static_cast<int32_t>(off)),
+                "half heights");
+        TVS_HIP(c, hipStreamSynchronize(c->stream), "half heights");
+        c->h16_offset = static_cast<float>(off);
+        c->has_h16 = true;
     }
     return upload_common(c, true, false, lo, hi);
 }
@@ -639,6 +658,19 @@ tvs_status tvs_compute_bands(tvs_ctx *c, uint32_t first, uint32_t count) {
 tvs_status tvs_set_short_divide(tvs_ctx *c, uint32_t enable) {
     if (!c) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null context");
     c->short_div = enable != 0;
+    return TVS_OK;
+}
+
+tvs_status tvs_set_half_heights(tvs_ctx *c, uint32_t enable) {
+    if (!c) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null context");
+    c->half_heights = enable != 0;
+    return TVS_OK;
+}
+
+tvs_status tvs_get_half_heights(const tvs_ctx *c, uint32_t *enabled, uint32_t *available) {
+    if (!c || !enabled || !available) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null argument");
+    *enabled = c->half_heights ? 1u : 0u;
+    *available = c->has_i16 && c->has_h16 ? 1u : 0u;
     return TVS_OK;
 }
 

@@ -368,6 +368,18 @@ tvs_status tvs_batch_set_short_divide(tvs_batch *b, uint32_t enable) {
     return TVS_OK;
 }
 
+tvs_status tvs_batch_set_half_heights(tvs_batch *b, uint32_t enable) {
+    if (!b) return bfail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null batch");
+    const tvs_status q = quiesce(b);
+    if (q != TVS_OK) return q;
+    for (uint32_t i = 0; i < b->tiles.size(); ++i) {
+        const tvs_status st = tvs_set_half_heights(b->tiles[i], enable);
+        if (st != TVS_OK) return from_tile(b, i, st);
+    }
+    drop_graph(b);
+    return TVS_OK;
+}
+
 tvs_status tvs_batch_run(tvs_batch *b, uint32_t use_graph) {
     if (!b) return bfail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null batch");
     tvs_status st = prepare(b);

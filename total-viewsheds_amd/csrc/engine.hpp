@@ -119,6 +119,12 @@ struct TileDesc {
     uint32_t *ring_events;       // packed event ids (kRingCompact)
     const uint32_t *order;       // per sector slot, blocks_per_pass block indices in walk
                                  // order (null: ascending)
+    // int16 DEMs whose heights all lie within h16_offset +/- 2048: the same
+    // layout (both copies) as f16 values h - h16_offset, all exact (null:
+    // none).  The two-point walk decodes a height and subtracts the
+    // observer's in one mixed-precision FMA from these.
+    const void *elev_h16;
+    float h16_offset;
 };
 
 // Ring capture modes of the tile walk.
@@ -241,6 +247,13 @@ struct tvs_ctx {
     float *elev_f32 = nullptr;
     int16_t *elev_i16 = nullptr;
     bool has_f32 = false, has_i16 = false;
+    // The heights as f16 offsets from h16_offset (see TileDesc::elev_h16),
+    // when the int16 DEM's range allows; half_heights false turns their use
+    // off (the walk then converts the int16 heights; results are the same).
+    uint16_t *elev_h16 = nullptr;
+    bool has_h16 = false;
+    bool half_heights = true;
+    float h16_offset = 0.0f;
     // Height-difference bounds proven on the host for the fast divide.
     bool fast_div_ok = false;
     float min_abs_dh = 0.0f, max_abs_dh = 0.0f;
@@ -342,6 +355,8 @@ hipError_t launch_ring_offsets(hipStream_t s, const uint32_t *counts, uint32_t n
                                uint64_t *offsets);
 hipError_t launch_widen_i16(hipStream_t s, const int16_t *in, float *out,
                             size_t n);
+hipError_t launch_half_heights(hipStream_t s, const int16_t *in, uint16_t *out, size_t n,
+                               int32_t offset);
 
 // ---- host side of a run (api.cpp) ----
 // tvs_run_sectors; with keep_rings the run records no rings and leaves the

@@ -175,11 +175,15 @@ def load() -> ctypes.CDLL:
                                    ctypes.c_size_t, P(ctypes.c_uint64)]),
         "tvs_set_band_method": (st, [vp, ctypes.c_int]),
         "tvs_set_short_divide": (st, [vp, ctypes.c_uint32]),
+        "tvs_set_half_heights": (st, [vp, ctypes.c_uint32]),
+        "tvs_get_half_heights": (st, [vp, ctypes.POINTER(ctypes.c_uint32),
+                                      ctypes.POINTER(ctypes.c_uint32)]),
         "tvs_get_short_divide": (st, [vp, P(ctypes.c_uint32), P(ctypes.c_uint32)]),
         "tvs_get_fast_divide": (st, [vp, P(ctypes.c_uint32)]),
         "tvs_get_walk_history": (st, [vp, P(ctypes.c_float), ctypes.c_size_t,
                                       P(ctypes.c_size_t)]),
         "tvs_batch_set_short_divide": (st, [vp, ctypes.c_uint32]),
+        "tvs_batch_set_half_heights": (st, [vp, ctypes.c_uint32]),
         "tvs_bands_sorted_count": (st, [vp, P(ctypes.c_uint32)]),
         "tvs_set_walk_shape": (st, [vp, P(WalkShape)]),
         "tvs_get_walk_shape": (st, [vp, P(WalkShape)]),
@@ -427,6 +431,15 @@ class Engine:
     def set_short_divide(self, enable: bool) -> None:
         self._c(load().tvs_set_short_divide(self._h, int(enable)))
 
+    def set_half_heights(self, enable: bool) -> None:
+        self._c(load().tvs_set_half_heights(self._h, int(enable)))
+
+    def half_heights(self) -> tuple[bool, bool]:
+        """(enabled, the uploaded DEM has the f16 form)."""
+        en, av = ctypes.c_uint32(0), ctypes.c_uint32(0)
+        self._c(load().tvs_get_half_heights(self._h, ctypes.byref(en), ctypes.byref(av)))
+        return bool(en.value), bool(av.value)
+
     def short_divide(self) -> tuple[bool, int]:
         """(enabled, sectors verified for the two-operation divide)."""
         en, n = ctypes.c_uint32(0), ctypes.c_uint32(0)
@@ -534,6 +547,10 @@ class Batch:
 
     def set_short_divide(self, enable: bool) -> None:
         self._c(load().tvs_batch_set_short_divide(self._h, int(enable)))
+
+    def set_half_heights(self, enable: bool) -> None:
+        """Engine.set_half_heights for every tile."""
+        self._c(load().tvs_batch_set_half_heights(self._h, int(enable)))
 
     def autotune(self, reps: int = 1):
         """Time every compiled launch shape on the whole batch, keep the
