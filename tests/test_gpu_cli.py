@@ -59,7 +59,7 @@ def test_cli_heatmap(tmp_path, small, method):
 def test_cli_tune_caches_the_walk_shape(tmp_path, small):
     """--tune times the walk's launch shapes on the DEM and keeps the
     fastest; the choice is written under the state directory per width,
-    line of sight and height type, and the next run loads it instead of
+    line of sight in points (6700 m at 100 m: 67) and height type, and the next run loads it instead of
     tuning again.  Both runs draw the oracle's picture."""
     wl, dem, cum = small
     path = tmp_path / "dem.bt"
@@ -72,7 +72,7 @@ def test_cli_tune_caches_the_walk_shape(tmp_path, small):
     r = subprocess.run(args, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     assert "Tuning the walk's launch shape" in r.stderr
-    shape_file = state / "total-viewsheds" / "201" / "walk_shape" / "6700_i16.txt"
+    shape_file = state / "total-viewsheds" / "201" / "walk_shape" / "67pts_i16.txt"
     fields = shape_file.read_text().split()
     assert len(fields) == 7 and int(fields[0]) in (64, 128, 256, 512)
     np.testing.assert_array_equal(read_png_grey(str(out / "heatmap.png")),

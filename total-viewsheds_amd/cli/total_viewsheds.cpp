@@ -126,12 +126,14 @@ bool make_dirs(const std::string &path) {
 }
 
 // The walk shape chosen by --tune for one geometry and height type, under
-// <state>/total-viewsheds/<width>/walk_shape/<line of sight>_<i16|f32>.txt
-// as the seven integers of tvs_walk_shape (block, unroll, xcd_contiguous,
+// <state>/total-viewsheds/<width>/walk_shape/<L>pts_<i16|f32>.txt (L the
+// line of sight in points, which with the width fixes the walk's work) as
+// the seven integers of tvs_walk_shape (block, unroll, xcd_contiguous,
 // pipeline, persistent, both_directions, two_points).
-std::string shape_path(const std::string &state, uint32_t width, uint32_t los, bool is_float) {
+std::string shape_path(const std::string &state, uint32_t width, uint32_t los_points,
+                       bool is_float) {
     return state + "/total-viewsheds/" + std::to_string(width) + "/walk_shape/" +
-           std::to_string(los) + (is_float ? "_f32" : "_i16") + ".txt";
+           std::to_string(los_points) + (is_float ? "pts_f32" : "pts_i16") + ".txt";
 }
 
 bool load_shape(const std::string &path, tvs_walk_shape *out) {
@@ -247,7 +249,8 @@ int main(int argc, char **argv) {
     if (cfg.tune && method == TVS_METHOD_BATCHED) {
         // A cached choice for this geometry and height type, else the tuner
         // (it keeps the fastest shape as the context's; surfaces untouched).
-        const std::string sp = state.empty() ? "" : shape_path(state, hdr.width, los, hdr.is_float);
+        const std::string sp =
+            state.empty() ? "" : shape_path(state, hdr.width, g.max_los_as_points, hdr.is_float);
         tvs_walk_shape best{};
         if (cfg.use_cache && !sp.empty() && load_shape(sp, &best) &&
             tvs_set_walk_shape(ctx, &best) == TVS_OK) {
