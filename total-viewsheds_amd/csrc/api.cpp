@@ -320,6 +320,7 @@ tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t cap
                         : static_cast<const void *>(c->elev_f32);
     d.span = static_cast<uint32_t>(dem_span(w2));
     d.copy_offset = static_cast<uint32_t>(dem_copy_offset(w2));
+    d.pov_shift = pair_parity(c->geo.dem_width, c->geo.max_los_as_points);
     d.elev_h16 = c->has_i16 && c->has_h16 && c->half_heights ? c->elev_h16 : nullptr;
     d.h16_offset = c->h16_offset;
     d.off = c->tab.off;

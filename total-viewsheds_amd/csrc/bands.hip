@@ -317,16 +317,21 @@ __global__ __launch_bounds__(kScanThreads) void step_table_kernel(
         fast = n_steps;
         for (uint32_t wv = 0; wv < kScanThreads / 64; ++wv) fast = min(fast, red_fast2[wv]);
     }
-    const int32_t bias = max(hi, -lo);
-    // Byte offsets of int16 elements: 2 * (bias +/- off), plus 2 * copy_offset
-    // (copy B; 0 when the DEM has none) where a two-point pair's element
-    // pov +/- off is odd (pov has the pairs' parity; off and -off have one
-    // parity); the host keeps them below 2^32.
+    // Even, so that the buffer base 2 * bias bytes below the DEM and every
+    // position below are dword-aligned wherever the element they reach is.
+    int32_t bias = max(hi, -lo);
+    bias += bias & 1;
+    // Byte offsets of int16 elements: 2 * (bias + parity +/- off), plus
+    // 2 * copy_offset (copy B; 0 when the DEM has none) where a two-point
+    // pair's element pov +/- off is odd (pov has the pairs' parity; off and
+    // -off have one parity).  The walks' vector offset is that of element
+    // pov - parity, so the sum reaches pov +/- off, and both offsets of a
+    // two-point load are dword-aligned.  The host keeps them below 2^32.
     for (uint32_t i = begin; i < end; ++i) {
         const int32_t o = tab.off[tb + i];
         const uint32_t b = ((static_cast<uint32_t>(o) + parity) & 1u) ? 2u * copy_offset : 0u;
-        tab.pos_f[tb + i] = 2u * static_cast<uint32_t>(bias + o) + b;
-        tab.pos_b[tb + i] = 2u * static_cast<uint32_t>(bias - o) + b;
+        tab.pos_f[tb + i] = 2u * (static_cast<uint32_t>(bias + o) + parity) + b;
+        tab.pos_b[tb + i] = 2u * (static_cast<uint32_t>(bias - o) + parity) + b;
     }
     if (tid == 0) {
         SectorMeta m{};

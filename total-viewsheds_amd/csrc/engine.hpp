@@ -44,6 +44,15 @@ inline uint64_t dem_span(uint64_t n) { return dem_copy_offset(n) + n; }
 // Parity of the DEM index of every two-point pair's first point: pairs
 // start at even columns, and on odd-width DEMs odd rows start at column 1,
 // which makes the parity 0; on even-width DEMs it is that of L.
+//
+// A raw dword buffer load runs at the texture unit's misaligned rate (about
+// twice the time) whenever its vector offset or its scalar offset is 2 mod
+// 4, even if their sum is aligned (measured on gfx950: 3.6 against 6.7 ns
+// per wave64 load per CU).  So the walks keep both aligned: a lane's
+// vector offset is that of element pov - pair_parity (TileDesc::pov_shift)
+// and every step-table position carries the other element, and each
+// sector's bias (the step offsets' lower bound, which places the buffer's
+// base below the DEM) is even.
 inline uint32_t pair_parity(uint32_t w, uint32_t L) { return (w & 1u) ? 0u : (L & 1u); }
 
 // Steps of one sector's walk are split in four ranges, each a uniform
@@ -107,6 +116,8 @@ struct TileDesc {
     uint32_t N, T, L, w;
     uint32_t span;               // elements from elev to the end of copy B
     uint32_t copy_offset;        // first element of copy B
+    uint32_t pov_shift;          // pair_parity(w, L): a lane's vector offset is that of
+                                 // element pov - pov_shift, the step tables add it back
     float obs;
     uint32_t blocks_per_pass;    // ceil(N / block)
     uint32_t R, capture_slots;

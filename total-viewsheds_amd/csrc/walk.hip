@@ -301,7 +301,7 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
     const H *elev = static_cast<const H *>(d.elev);
     const uint32_t slot = d.first_slot + rel;
     const uint32_t pov = (t / d.T + d.L) * d.w + (t % d.T + d.L);
-    const uint32_t pov_bytes = pov * static_cast<uint32_t>(sizeof(H));
+    const uint32_t pov_bytes = (pov - d.pov_shift) * static_cast<uint32_t>(sizeof(H));
     const float h0 = static_cast<float>(global_ptr(elev)[pov]) + d.obs;
     const uint32_t head = const_ptr(d.meta)[slot].head;
     const uint32_t n_fast = const_ptr(d.meta)[slot].n_fast;
@@ -350,11 +350,12 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
     }
     auto event = [&](uint32_t p) {
         ++ring_id;
-        // p / 2 is bias +/- off, plus copy_offset for a copy-B step (bias +/-
-        // off <= 2 bias < copy_offset; no step goes there when the offset
-        // is 0).
+        // p / 2 is bias + pov_shift +/- off, plus copy_offset for a copy-B
+        // step (bias +/- off <= 2 bias < copy_offset; no step goes there
+        // when the offset is 0).
         const uint32_t e = p >> 1;
-        const uint32_t id = pov + (e > 2u * bias ? e - d.copy_offset : e) - bias;   // pov +/- off
+        const uint32_t id = pov - d.pov_shift +
+                            (e > 2u * bias + d.pov_shift ? e - d.copy_offset : e) - bias;   // pov +/- off
         if (d.ring_mode == kRingRecords) {
             if (ring_id < d.R) band_ring[ring_id] = id; else ++dropped;
         } else if (d.ring_mode == kRingCompact) {
@@ -452,7 +453,7 @@ __device__ __forceinline__ void walk_band(const TileDesc &d, uint32_t rel, bool 
         }
     }
     if (CAPTURE && capture) {
-        if (visible) event(d.n_steps ? pos[d.n_steps - 1] : 2u * bias);
+        if (visible) event(d.n_steps ? pos[d.n_steps - 1] : 2u * (bias + d.pov_shift));
         if (d.ring_mode == kRingRecords) {
             if (d.R > 0) band_ring[0] = ring_id;
             if (dropped) atomicAdd(d.overflow, static_cast<unsigned long long>(dropped));
@@ -477,7 +478,7 @@ __device__ __forceinline__ void walk_pair(const TileDesc &d, uint32_t rel, uint3
     const H *elev = static_cast<const H *>(d.elev);
     const uint32_t slot = d.first_slot + rel;
     const uint32_t pov = (t / d.T + d.L) * d.w + (t % d.T + d.L);
-    const uint32_t pov_bytes = pov * static_cast<uint32_t>(sizeof(H));
+    const uint32_t pov_bytes = (pov - d.pov_shift) * static_cast<uint32_t>(sizeof(H));
     const float h0 = static_cast<float>(global_ptr(elev)[pov]) + d.obs;
     const uint32_t head = const_ptr(d.meta)[slot].head;
     const uint32_t n_fast = const_ptr(d.meta)[slot].n_fast;
@@ -587,7 +588,7 @@ __device__ __forceinline__ void walk_twin(const TileDesc &d, uint32_t rel, bool 
     const H *elev = static_cast<const H *>(d.elev);
     const uint32_t slot = d.first_slot + rel;
     const uint32_t pov = (t0 / d.T + d.L) * d.w + (t0 % d.T + d.L);
-    const uint32_t pov_bytes = pov * static_cast<uint32_t>(sizeof(H));
+    const uint32_t pov_bytes = (pov - d.pov_shift) * static_cast<uint32_t>(sizeof(H));
     const float h0a = static_cast<float>(global_ptr(elev)[pov]) + d.obs;
     const float h0b = static_cast<float>(global_ptr(elev)[pov + 1u]) + d.obs;
     const uint32_t head = const_ptr(d.meta)[slot].head;
@@ -750,8 +751,8 @@ __device__ __forceinline__ void walk_quad(const TileDesc &d, uint32_t rel, uint3
     const uint32_t slot = d.first_slot + rel;
     const uint32_t pov_a = (ta / d.T + d.L) * d.w + (ta % d.T + d.L);
     const uint32_t pov_b = (tb / d.T + d.L) * d.w + (tb % d.T + d.L);
-    const uint32_t bytes_a = pov_a * static_cast<uint32_t>(sizeof(H));
-    const uint32_t bytes_b = pov_b * static_cast<uint32_t>(sizeof(H));
+    const uint32_t bytes_a = (pov_a - d.pov_shift) * static_cast<uint32_t>(sizeof(H));
+    const uint32_t bytes_b = (pov_b - d.pov_shift) * static_cast<uint32_t>(sizeof(H));
     float h0[4];
     h0[0] = static_cast<float>(global_ptr(elev)[pov_a]) + d.obs;
     h0[1] = static_cast<float>(global_ptr(elev)[pov_a + 1u]) + d.obs;
