@@ -762,3 +762,22 @@ def test_directional_order_shape_round_trip(small):
         with pytest.raises(tvs_amd.TvsError) as err:
             e.set_walk_shape(64, 8, 3, False, False, True, True)
         assert err.value.status == _lib.ERR_INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("w,heights,directional", [(2896, "int16", False),
+                                                   (2897, "int16", True),
+                                                   (2897, "f32 fraction", False)])
+def test_default_item_order_by_dem_size(w, heights, directional):
+    """Untuned int16 DEMs above 2^23 points (2897^2; 2896^2 is below) walk
+    in the directional item order by default, smaller ones and f32 heights
+    in the XCD-contiguous order; a set shape overrides it."""
+    elev = np.random.default_rng(w).integers(0, 2000, w * w).astype(np.int16)
+    if heights != "int16":
+        elev = elev.astype(np.float32) + np.float32(0.25)
+    with tvs_amd.Engine(w, 30.0, (w // 3) * 30, 5.0, 1.65, 0) as e:
+        e.upload(elev)
+        shape = e.walk_shape()
+        note(f"default shape {w}x{w} {heights}: {shape}")
+        assert shape["xcd_contiguous"] and shape["directional"] == directional
+        e.set_walk_shape(64, 8, 1, False, False, True, 1)
+        assert not e.walk_shape()["directional"]

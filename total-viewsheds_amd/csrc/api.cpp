@@ -261,7 +261,11 @@ tvs_status validate_slots(tvs_ctx *c, uint32_t first, uint32_t count) {
 }
 
 WalkConfig effective_walk_config(const tvs_ctx *c) {
-    return c->walk_cfg_set ? c->walk_cfg : default_walk_config(c->has_i16, true);
+    if (c->walk_cfg_set) return c->walk_cfg;
+    WalkConfig cfg = default_walk_config(c->has_i16, true);
+    cfg.directional = c->has_i16 && static_cast<uint64_t>(c->geo.dem_width) *
+                                            c->geo.dem_width > kDirectionalDefaultPoints;
+    return cfg;
 }
 
 // Verifies the short divide for the unchecked slots of [first, first +

@@ -178,6 +178,14 @@ inline WalkConfig default_walk_config(bool i16, bool xcd_contiguous) {
                : WalkConfig{128, 8, xcd_contiguous, false, false, true, false};
 }
 
+// Untuned single DEMs with int16 heights above this many points walk in the
+// directional item order.  Measured per pass, 64 x 8 two-point shape, same
+// box, XCD-contiguous against directional: 1201^2 to 2400^2 within +-2 %;
+// 3072^2 30.6 against 28.9 ms; 3601^2 35.1 against 33.6; 4096^2 (heights
+// spanning 2900 m) 108.6 against 68.1, where the contiguous runs of whole
+// sectors that each XCD takes overflow its L2 with the DEM's two copies.
+constexpr uint64_t kDirectionalDefaultPoints = uint64_t(1) << 23;
+
 // The one shape of ring-capturing walks (a checking path): 256 points per
 // block, 8 steps per chunk, one band per lane.
 inline WalkConfig capture_walk_config(const WalkConfig &cfg) {
