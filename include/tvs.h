@@ -343,9 +343,10 @@ typedef struct tvs_walk_shape {
 
 /* Until a shape is set or tuned, batched runs use the fastest shape measured
  * for the uploaded height type (int16: 64 lanes x 8 steps, two points and
- * both directions per lane, XCD-contiguous items, in the directional order
- * (xcd_contiguous 2) above 2^23 points; f32: 128 x 8, both directions, one
- * point per lane, XCD-contiguous);
+ * both directions per lane; f32: 128 x 8, both directions, one point per
+ * lane), with XCD-contiguous items, in the directional order
+ * (xcd_contiguous 2) when the heights take more than 16 MiB (int16 DEMs
+ * above 2^23 points, f32 above 2^22);
  * tvs_get_walk_shape reports the shape the next run uses. */
 tvs_status tvs_set_walk_shape(tvs_ctx *ctx, const tvs_walk_shape *shape);
 tvs_status tvs_get_walk_shape(const tvs_ctx *ctx, tvs_walk_shape *out);

@@ -766,11 +766,13 @@ def test_directional_order_shape_round_trip(small):
 
 @pytest.mark.parametrize("w,heights,directional", [(2896, "int16", False),
                                                    (2897, "int16", True),
-                                                   (2897, "f32 fraction", False)])
+                                                   (2048, "f32 fraction", False),
+                                                   (2049, "f32 fraction", True)])
 def test_default_item_order_by_dem_size(w, heights, directional):
-    """Untuned int16 DEMs above 2^23 points (2897^2; 2896^2 is below) walk
-    in the directional item order by default, smaller ones and f32 heights
-    in the XCD-contiguous order; a set shape overrides it."""
+    """Untuned DEMs whose heights take more than 16 MiB (int16 above 2^23
+    points: 2897^2, not 2896^2; f32 above 2^22: 2049^2, not 2048^2) walk in
+    the directional item order by default, smaller ones in the
+    XCD-contiguous order; a set shape overrides it."""
     elev = np.random.default_rng(w).integers(0, 2000, w * w).astype(np.int16)
     if heights != "int16":
         elev = elev.astype(np.float32) + np.float32(0.25)

@@ -263,8 +263,8 @@ tvs_status validate_slots(tvs_ctx *c, uint32_t first, uint32_t count) {
 WalkConfig effective_walk_config(const tvs_ctx *c) {
     if (c->walk_cfg_set) return c->walk_cfg;
     WalkConfig cfg = default_walk_config(c->has_i16, true);
-    cfg.directional = c->has_i16 && static_cast<uint64_t>(c->geo.dem_width) *
-                                            c->geo.dem_width > kDirectionalDefaultPoints;
+    cfg.directional = static_cast<uint64_t>(c->geo.dem_width) * c->geo.dem_width *
+                          (c->has_i16 ? 2u : 4u) > kDirectionalDefaultBytes;
     return cfg;
 }
 

@@ -178,13 +178,17 @@ inline WalkConfig default_walk_config(bool i16, bool xcd_contiguous) {
                : WalkConfig{128, 8, xcd_contiguous, false, false, true, false};
 }
 
-// Untuned single DEMs with int16 heights above this many points walk in the
-// directional item order.  Measured per pass, 64 x 8 two-point shape, same
-// box, XCD-contiguous against directional: 1201^2 to 2400^2 within +-2 %;
-// 3072^2 30.6 against 28.9 ms; 3601^2 35.1 against 33.6; 4096^2 (heights
-// spanning 2900 m) 108.6 against 68.1, where the contiguous runs of whole
-// sectors that each XCD takes overflow its L2 with the DEM's two copies.
-constexpr uint64_t kDirectionalDefaultPoints = uint64_t(1) << 23;
+// Untuned single DEMs whose heights (one copy, as stored: int16 or f32)
+// take more than this many bytes walk in the directional item order: 2^23
+// points with int16 heights, 2^22 with f32.  Measured per pass, same box,
+// XCD-contiguous against directional.  int16, 64 x 8 two-point shape:
+// 1201^2 to 2400^2 within +-2 %; 3072^2 30.6 against 28.9 ms; 3601^2 35.1
+// against 33.6; 4096^2 (heights spanning 2900 m) 108.6 against 68.1.  f32,
+// 128 x 8 one-point shape: 1201^2 3.25 against 3.24 ms; 2401^2 28.4
+// against 22.9; 3601^2 182.7 against 81.7.  The contiguous runs of whole
+// sectors that each XCD takes keep far more of a large DEM in flight than
+// its L2 holds.
+constexpr uint64_t kDirectionalDefaultBytes = uint64_t(1) << 24;
 
 // The one shape of ring-capturing walks (a checking path): 256 points per
 // block, 8 steps per chunk, one band per lane.
