@@ -149,9 +149,10 @@ def test_autotune_keeps_results_exact(tiles_and_truth):
         before = e.surfaces()
         best, table = e.autotune(0, 30, reps=1)
         print("autotune", best, flush=True)
-        # every compiled shape in both item orders, the two-point shapes also
-        # in the directional one, persistent shapes once
-        assert len(table) == _lib.load().tvs_walk_shape_count() == 2 * 23 + 5 + 7
+        # every compiled shape in both item orders, the two-point shapes and
+        # the one-point shapes walking both directions also in the
+        # directional one, persistent shapes once
+        assert len(table) == _lib.load().tvs_walk_shape_count() == 2 * 23 + 5 + 12
         assert best["ms"] > 0
         np.testing.assert_array_equal(e.surfaces(), before)   # surfaces preserved
         assert e.walk_shape()["block"] == best["block"]
@@ -173,7 +174,7 @@ def test_batch_autotune_keeps_results_exact(tiles_and_truth):
         before = [b.surfaces(i) for i in range(len(tiles))]
         best, table = b.autotune(reps=1)
         print("batch autotune", best, flush=True)
-        assert len(table) == _lib.load().tvs_walk_shape_count() == 2 * 23 + 5 + 7
+        assert len(table) == _lib.load().tvs_walk_shape_count() == 2 * 23 + 5 + 12
         assert best["ms"] > 0
         for i in range(len(tiles)):
             np.testing.assert_array_equal(b.surfaces(i), before[i])

@@ -21,10 +21,12 @@ std::vector<WalkConfig> compiled_walk_shapes() {
     std::vector<WalkConfig> out;
     // A persistent grid strides over the items itself: one item mapping.
     // The directional order is timed for the two-points-per-lane shapes
-    // (the int16 walk's fastest family).
+    // (the int16 walk's fastest family) and the one-point shapes that walk
+    // both directions (the f32 walk's), which gain most from it on large
+    // DEMs.
 #define TVS_SHAPE(B, U, P, S, D, W)                                                  \
     if (!S) out.push_back(WalkConfig{B, U, true, P, S, D, W});                  \
-    if (!S && W) out.push_back(WalkConfig{B, U, true, P, S, D, W, true});       \
+    if (!S && (W || D)) out.push_back(WalkConfig{B, U, true, P, S, D, W, true}); \
     out.push_back(WalkConfig{B, U, false, P, S, D, W});
     TVS_WALK_SHAPES(TVS_SHAPE)
 #undef TVS_SHAPE
