@@ -297,6 +297,16 @@ tvs_status tvs_ring_overflow_count(tvs_ctx *ctx, uint64_t *out);
 /* Wait for the context's stream. */
 tvs_status tvs_sync(tvs_ctx *ctx);
 
+/* Page-locked host memory (hipHostMalloc): device-to-host copies into it
+ * (the counts and events of tvs_compact_rings and tvs_observer_walk,
+ * surfaces, ring records) go straight over the link instead of through the
+ * runtime's pageable staging.  NULL when `bytes` is 0 or the allocation
+ * fails.  Free with tvs_host_free (NULL is accepted).  Stands for the
+ * reference's MAP_READ download buffers (gpu.rs:227-248), which wgpu maps
+ * for the host to read. */
+void *tvs_host_alloc(size_t bytes);
+tvs_status tvs_host_free(void *ptr);
+
 /* Device times of the last run and band computation. */
 tvs_status tvs_get_timing(tvs_ctx *ctx, tvs_timing *out);
 

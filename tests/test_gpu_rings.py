@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import tvs_amd
-from tvs_amd import demgen, rings
+from tvs_amd import demgen, rings, _lib
 
 pytestmark = pytest.mark.gpu
 THREADS = min(16, os.cpu_count() or 1)
@@ -124,3 +124,29 @@ def test_compact_rings_at_1201_equal_fixed_slots():
         R = e.rings_per_band
         np.testing.assert_array_equal(rings.to_records(e.geometry, counts, events, R),
                                       e.rings(0))
+
+
+def test_compact_rings_into_page_locked_memory():
+    """tvs_host_alloc memory as the export's destination: the counts and
+    events equal those copied into ordinary (pageable) arrays, and the
+    allocation outlives the slices taken of it.  tvs_host_alloc(0) is NULL
+    and tvs_host_free(NULL) succeeds."""
+    L = _lib.load()
+    assert not L.tvs_host_alloc(0)
+    assert L.tvs_host_free(None) == 0
+    wl = demgen.main_1201()
+    with tvs_amd.Engine(wl.width, wl.spacing, wl.max_line_of_sight, wl.rings_per_km,
+                        wl.observer_height, 0) as e:
+        e.upload(wl.elevations)
+        e.compute_bands(46, 2)
+        for s in (46, 47):
+            print(f"compact rings 1201 sector {s}: page-locked and pageable", flush=True)
+            pc, pe = e.compact_rings(s, pinned=True)
+            qc, qe = e.compact_rings(s, pinned=False)
+            assert pc.flags.writeable and pe.size == qe.size
+            np.testing.assert_array_equal(pc, qc)
+            np.testing.assert_array_equal(pe, qe)
+    # The arrays are views of page-locked memory that they keep alive.
+    import gc
+    gc.collect()
+    assert int(pc.sum()) == int(qc.sum()) and int(pe[-1]) == int(qe[-1])

@@ -1057,6 +1057,21 @@ tvs_status tvs_sync(tvs_ctx *c) {
     return TVS_OK;
 }
 
+void *tvs_host_alloc(size_t bytes) {
+    if (bytes == 0) return nullptr;
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();   // leave no sticky error for the next call
+        return nullptr;
+    }
+    return p;
+}
+
+tvs_status tvs_host_free(void *p) {
+    if (!p) return TVS_OK;
+    return hipHostFree(p) == hipSuccess ? TVS_OK : TVS_ERR_HIP;
+}
+
 tvs_status tvs_get_timing(tvs_ctx *c, tvs_timing *out) {
     if (!c || !out) return fail(c, TVS_ERR_INVALID_ARGUMENT, "null argument");
     if (c->timing.walk_launches) {

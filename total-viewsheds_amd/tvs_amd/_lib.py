@@ -171,6 +171,8 @@ def load() -> ctypes.CDLL:
         "tvs_band_cache_save": (st, [vp, ctypes.c_char_p, ctypes.c_uint32]),
         "tvs_band_cache_load": (st, [vp, ctypes.c_char_p, ctypes.c_uint32]),
         "tvs_io_last_error": (ctypes.c_char_p, []),
+        "tvs_host_alloc": (vp, [ctypes.c_size_t]),
+        "tvs_host_free": (st, [vp]),
         "tvs_compact_rings": (st, [vp, ctypes.c_uint32, vp, ctypes.c_size_t, vp,
                                    ctypes.c_size_t, P(ctypes.c_uint64)]),
         "tvs_set_band_method": (st, [vp, ctypes.c_int]),
@@ -256,6 +258,32 @@ class RunTiming:
     bands_ms: float
     walk_launches: int
     point_steps: int
+
+
+class _HostBuffer:
+    """Page-locked host memory from tvs_host_alloc, freed with the last
+    array that views it."""
+
+    def __init__(self, nbytes: int):
+        self.ptr = load().tvs_host_alloc(nbytes)
+        if not self.ptr:
+            raise MemoryError(f"tvs_host_alloc({nbytes}) failed")
+
+    def __del__(self):
+        ptr, self.ptr = getattr(self, "ptr", None), None
+        if ptr and _lib is not None:       # (at interpreter exit the module may be gone)
+            _lib.tvs_host_free(ptr)
+
+
+def pinned_empty(n: int, dtype) -> np.ndarray:
+    """An uninitialised array of n elements in page-locked host memory, so
+    that device-to-host copies into it run at the link's DMA rate."""
+    dtype = np.dtype(dtype)
+    nbytes = max(n, 1) * dtype.itemsize
+    owner = _HostBuffer(nbytes)
+    raw = (ctypes.c_uint8 * nbytes).from_address(owner.ptr)
+    raw._owner = owner            # the array's base keeps the allocation alive
+    return np.frombuffer(raw, dtype=dtype, count=max(n, 1))
 
 
 class Engine:
@@ -383,15 +411,18 @@ class Engine:
         self._c(load().tvs_read_rings(self._h, slot, _ptr(out), n))
         return out[:n]
 
-    def compact_rings(self, sector: int):
+    def compact_rings(self, sector: int, pinned: bool = True):
         """(counts u32[2N], events u32[total]) of one sector: counts[k] is the
-        count slot of band k, events its ids after the PoV slot."""
+        count slot of band k, events its ids after the PoV slot.  With
+        `pinned` both arrays live in page-locked host memory
+        (tvs_host_alloc), which the device writes at DMA rate."""
+        empty = pinned_empty if pinned else (lambda n, t: np.empty(max(n, 1), t))
         bands = 2 * self.n_points
-        counts = np.empty(max(bands, 1), np.uint32)
+        counts = empty(bands, np.uint32)
         total = ctypes.c_uint64(0)
         self._c(load().tvs_compact_rings(self._h, sector, _ptr(counts), bands, None, 0,
                                          ctypes.byref(total)))
-        events = np.empty(max(int(total.value), 1), np.uint32)
+        events = empty(int(total.value), np.uint32)
         self._c(load().tvs_compact_rings(self._h, sector, _ptr(counts), bands, _ptr(events),
                                          events.size, ctypes.byref(total)))
         return counts[:bands], events[:int(total.value)]
