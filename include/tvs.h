@@ -396,6 +396,11 @@ tvs_status tvs_batch_upload_f32(tvs_batch *batch, uint32_t tile,
 /* Band data for all 180 sectors of every distinct (width, band size). */
 tvs_status tvs_batch_compute_bands(tvs_batch *batch);
 tvs_status tvs_batch_set_walk_shape(tvs_batch *batch, const tvs_walk_shape *shape);
+/* The shape the batch's next run uses: the set or tuned one, else the
+ * fastest measured for the tiles' height type with items round-robin over
+ * the XCDs, in the directional order (xcd_contiguous 2) when a tile's
+ * heights take more than 16 MiB. */
+tvs_status tvs_batch_get_walk_shape(const tvs_batch *batch, tvs_walk_shape *out);
 /* tvs_set_short_divide for every tile. */
 tvs_status tvs_batch_set_short_divide(tvs_batch *batch, uint32_t enable);
 /* tvs_set_half_heights for every tile. */

@@ -270,3 +270,38 @@ def test_autotune_leaves_ring_capture_alone(tiles_and_truth):
         np.testing.assert_array_equal(e.rings(0), rings_before)
         assert e.ring_overflow() == overflow_before
         assert len(table) == _lib.load().tvs_walk_shape_count() and best["ms"] > 0
+
+
+@pytest.mark.parametrize("sides,directional", [((257, 401), False), ((2897, 257), True)])
+def test_batch_default_item_order_by_tile_size(sides, directional):
+    """An untuned batch walks its items round-robin over the XCDs, or in the
+    directional order when a tile's int16 heights take more than 16 MiB
+    (2897^2 points).  Either way every tile's TVS equals its own context's
+    run in the XCD-contiguous order, bit for bit; a set shape overrides the
+    default."""
+    elev = [np.random.default_rng(w).integers(0, 2000, w * w).astype(np.int16) for w in sides]
+    geo = [(w, 30.0, (w // 3) * 30, 5.0, 1.65) for w in sides]
+    want = []
+    for g, h in zip(geo, elev):
+        with tvs_amd.Engine(*g, 0) as e:
+            e.upload(h)
+            e.compute_bands(0, 180)
+            e.set_walk_shape(64, 8, 1, False, False, True, 1)
+            e.run(0, 180, "batched")
+            want.append(e.surfaces())
+    with tvs_amd.Batch(geo) as b:
+        for i, h in enumerate(elev):
+            b.upload(i, h)
+        shape = b.walk_shape()
+        print(f"batch {sides}: default shape {shape}", flush=True)
+        assert shape["directional"] == directional
+        assert shape["xcd_contiguous"] == directional
+        b.compute_bands()
+        b.run(True)
+        b.sync()
+        for i in range(len(sides)):
+            np.testing.assert_array_equal(b.surfaces(i).view(np.uint32),
+                                          want[i].view(np.uint32), err_msg=f"tile {i}")
+        b.set_walk_shape(64, 8, 1, both_directions=True, two_points=1)
+        s2 = b.walk_shape()
+        assert s2["xcd_contiguous"] and not s2["directional"]

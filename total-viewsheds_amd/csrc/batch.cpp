@@ -80,6 +80,33 @@ void drop_graph(tvs_batch *b) {
     b->prepared = false;
 }
 
+// The shape of an untuned batch: the fastest measured for its height type,
+// with its items round-robin over the XCDs (mixed item costs stay
+// balanced), or in the directional order when a tile's heights take more
+// than kDirectionalDefaultBytes.  Measured per launch, same box,
+// round-robin / XCD-contiguous / directional: three 3601 x 3601 int16
+// tiles 159.9 / 168.5 / 142.8 ms; sixteen 1201 x 1201 tiles 28.5 / 28.2 /
+// 28.7 ms.
+tvs::WalkConfig default_batch_config(const tvs_batch *b, bool i16) {
+    tvs::WalkConfig cfg = tvs::default_walk_config(i16, false);
+    uint64_t largest = 0;
+    for (const tvs_ctx *c : b->tiles)
+        largest = std::max<uint64_t>(largest,
+                                     static_cast<uint64_t>(c->geo.dem_width) * c->geo.dem_width);
+    if (largest * (i16 ? 2u : 4u) > tvs::kDirectionalDefaultBytes) {
+        cfg.xcd_contiguous = true;
+        cfg.directional = true;
+    }
+    return cfg;
+}
+
+// Every tile holds int16 heights (the launch then walks int16; else f32).
+bool all_tiles_i16(const tvs_batch *b) {
+    bool all = true;
+    for (const tvs_ctx *c : b->tiles) all = all && c->has_i16;
+    return all;
+}
+
 // Upload the descriptors and item/point prefixes for the current shape.
 tvs_status prepare(tvs_batch *b) {
     if (b->prepared) return TVS_OK;
@@ -92,8 +119,7 @@ tvs_status prepare(tvs_batch *b) {
         all_i16 = all_i16 && c->has_i16;
     }
     b->i16 = all_i16;
-    // Mixed item costs: round-robin item order keeps the XCDs balanced.
-    if (!b->cfg_set) b->cfg = tvs::default_walk_config(b->i16, false);
+    if (!b->cfg_set) b->cfg = default_batch_config(b, b->i16);
     std::vector<tvs::TileDesc> descs(n);
     std::vector<uint32_t> item_begin(n + 1, 0), point_begin(n + 1, 0);
     for (size_t k = 0; k < n; ++k) {
@@ -289,6 +315,20 @@ tvs_status tvs_batch_set_walk_shape(tvs_batch *b, const tvs_walk_shape *shape) {
     b->cfg = cfg;
     b->cfg_set = true;
     drop_graph(b);
+    return TVS_OK;
+}
+
+tvs_status tvs_batch_get_walk_shape(const tvs_batch *b, tvs_walk_shape *out) {
+    if (!b || !out) return TVS_ERR_INVALID_ARGUMENT;
+    const tvs::WalkConfig cfg = b->cfg_set ? b->cfg : default_batch_config(b, all_tiles_i16(b));
+    out->block = cfg.block;
+    out->unroll = cfg.unroll;
+    out->xcd_contiguous = tvs::xcd_mode(cfg);
+    out->pipeline = cfg.pipeline ? 1u : 0u;
+    out->persistent = cfg.persistent ? 1u : 0u;
+    out->both_directions = cfg.pair ? 1u : 0u;
+    out->two_points = cfg.twin;
+    out->ms = 0.0f;
     return TVS_OK;
 }
 

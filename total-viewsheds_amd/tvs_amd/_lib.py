@@ -199,6 +199,7 @@ def load() -> ctypes.CDLL:
         "tvs_batch_upload_f32": (st, [vp, ctypes.c_uint32, vp, ctypes.c_size_t]),
         "tvs_batch_compute_bands": (st, [vp]),
         "tvs_batch_set_walk_shape": (st, [vp, P(WalkShape)]),
+        "tvs_batch_get_walk_shape": (st, [vp, P(WalkShape)]),
         "tvs_batch_autotune": (st, [vp, ctypes.c_uint32, P(WalkShape), ctypes.c_size_t,
                                     P(ctypes.c_size_t), P(WalkShape)]),
         "tvs_batch_run": (st, [vp, ctypes.c_uint32]),
@@ -575,6 +576,11 @@ class Batch:
         self._c(load().tvs_batch_set_walk_shape(self._h, ctypes.byref(
             WalkShape(block, unroll, int(xcd_contiguous), int(pipeline), int(persistent),
                       int(both_directions), int(two_points), 0.0))))
+
+    def walk_shape(self) -> dict:
+        w = WalkShape()
+        self._c(load().tvs_batch_get_walk_shape(self._h, ctypes.byref(w)))
+        return w.as_dict()
 
     def set_short_divide(self, enable: bool) -> None:
         self._c(load().tvs_batch_set_short_divide(self._h, int(enable)))
