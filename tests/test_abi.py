@@ -32,6 +32,22 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_page_locked_memory_where_no_device_can_pin_it():
+    """tvs_host_alloc(0) is NULL and tvs_host_free(NULL) succeeds anywhere.
+    Without a device (this CPU-only run) every allocation is NULL and
+    pinned_empty raises MemoryError, which the ring export answers with
+    ordinary arrays."""
+    lib = _lib.load()
+    assert not lib.tvs_host_alloc(0)
+    assert lib.tvs_host_free(None) == 0
+    p = lib.tvs_host_alloc(64)
+    if p:                                   # a device is present: nothing to check here
+        assert lib.tvs_host_free(p) == 0
+        pytest.skip("a device can pin memory here")
+    with pytest.raises(MemoryError):
+        _lib.pinned_empty(16, np.uint32)
+
+
 @pytest.mark.parametrize("w,scale,los", [
     (9, 1.0, 3), (201, 100.0, 6700), (1201, 90.0, 36000), (3601, 30.0, 40000),
     (257, 30.0, 2550), (401, 90.0, 11970), (601, 30.0, 6000), (801, 90.0, 24030),

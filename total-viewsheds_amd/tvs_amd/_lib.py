@@ -416,8 +416,15 @@ class Engine:
         """(counts u32[2N], events u32[total]) of one sector: counts[k] is the
         count slot of band k, events its ids after the PoV slot.  With
         `pinned` both arrays live in page-locked host memory
-        (tvs_host_alloc), which the device writes at DMA rate."""
-        empty = pinned_empty if pinned else (lambda n, t: np.empty(max(n, 1), t))
+        (tvs_host_alloc), which the device writes at DMA rate; where that
+        memory cannot be had (a locked-memory limit), ordinary arrays."""
+        def empty(n, t):
+            if pinned:
+                try:
+                    return pinned_empty(n, t)
+                except MemoryError:
+                    pass
+            return np.empty(max(n, 1), t)
         bands = 2 * self.n_points
         counts = empty(bands, np.uint32)
         total = ctypes.c_uint64(0)
