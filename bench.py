@@ -354,14 +354,19 @@ def measure_next_rows(wl, shape: dict) -> dict:
             e.run(0, 180, "batched")
             e.sync()
             # Host to host, PCIe included: upload the DEM, band data for 180
-            # sectors, the walk, surfaces back (warm engine and shape).
-            t0 = time.perf_counter()
-            e.upload(wl.elevations)
-            e.compute_bands(0, 180)
-            e.reset_surfaces()
-            e.run(0, 180, "batched")
-            surf = e.surfaces()
-            res["host_to_host_ms"] = (time.perf_counter() - t0) * 1000.0
+            # sectors, the walk, surfaces back (warm engine and shape); the
+            # fastest of three passes, since one pass swings by ~20 % with
+            # the host's load.
+            h2h = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                e.upload(wl.elevations)
+                e.compute_bands(0, 180)
+                e.reset_surfaces()
+                e.run(0, 180, "batched")
+                surf = e.surfaces()
+                h2h.append((time.perf_counter() - t0) * 1000.0)
+            res["host_to_host_ms"] = min(h2h)
             T = e.geometry.tvs_width
             surf = surf.reshape(T, T)
             png = os.path.join(tmp, "heatmap.png")
