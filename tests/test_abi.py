@@ -32,6 +32,19 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_entry_points_open_marker_ranges():
+    """The library calls roctx (rocprofv3 --marker-trace shows a named range
+    per entry point call) and loads its runtime library."""
+    import shutil
+    import subprocess
+    if not shutil.which("nm"):
+        pytest.skip("no nm")
+    syms = subprocess.run(["nm", "-D", "--undefined-only", tvs_amd.LIB_PATH],
+                          capture_output=True, text=True, check=True).stdout
+    assert "roctxRangePushA" in syms and "roctxRangePop" in syms
+    ctypes.CDLL(tvs_amd.LIB_PATH)
+
+
 def test_page_locked_memory_where_no_device_can_pin_it():
     """tvs_host_alloc(0) is NULL and tvs_host_free(NULL) succeeds anywhere.
     Without a device (this CPU-only run) every allocation is NULL and

@@ -530,6 +530,7 @@ tvs_status tvs_get_geometry(const tvs_ctx *c, tvs_geometry *out) {
 }
 
 tvs_status tvs_upload_dem_f32(tvs_ctx *c, const float *elev, size_t count) {
+    tvs::TraceRange trace_range("tvs_upload_dem_f32");
     if (!c || !elev) return fail(c, TVS_ERR_INVALID_ARGUMENT, "null argument");
     if (count != c->geo.dem_size)
         return fail(c, TVS_ERR_SIZE, "elevation count " + std::to_string(count) +
@@ -575,6 +576,7 @@ tvs_status tvs_upload_dem_f32(tvs_ctx *c, const float *elev, size_t count) {
 }
 
 tvs_status tvs_upload_dem_i16(tvs_ctx *c, const int16_t *elev, size_t count) {
+    tvs::TraceRange trace_range("tvs_upload_dem_i16");
     if (!c || !elev) return fail(c, TVS_ERR_INVALID_ARGUMENT, "null argument");
     if (count != c->geo.dem_size)
         return fail(c, TVS_ERR_SIZE, "elevation count " + std::to_string(count) +
@@ -621,6 +623,7 @@ tvs_status tvs_upload_dem_i16(tvs_ctx *c, const int16_t *elev, size_t count) {
 }
 
 tvs_status tvs_compute_bands(tvs_ctx *c, uint32_t first, uint32_t count) {
+    tvs::TraceRange trace_range("tvs_compute_bands");
     if (!c) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null context");
     if (first >= tvs::kSectorSteps || count > tvs::kSectorSteps - first)
         return fail(c, TVS_ERR_INVALID_ARGUMENT, "sector range outside 0..180");
@@ -895,11 +898,13 @@ tvs_status run_sectors(tvs_ctx *c, uint32_t first, uint32_t count, tvs_method me
 extern "C" {
 
 tvs_status tvs_run_sectors(tvs_ctx *c, uint32_t first, uint32_t count, tvs_method method) {
+    tvs::TraceRange trace_range("tvs_run_sectors");
     return tvs::run_sectors(c, first, count, method, false);
 }
 
 tvs_status tvs_run_sector_with_band(tvs_ctx *c, const int32_t *deltas, const float *dists,
                                     size_t n, float *cum_out, size_t n_out) {
+    tvs::TraceRange trace_range("tvs_run_sector_with_band");
     if (!c) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null context");
     if (!c->has_f32) return fail(c, TVS_ERR_STATE, "no elevations uploaded");
     const uint32_t N = c->geo.computable_points_count;
@@ -927,6 +932,7 @@ tvs_status tvs_run_sector_with_band(tvs_ctx *c, const int32_t *deltas, const flo
 // surfaces of these walks are not reduced).
 tvs_status tvs_compact_rings(tvs_ctx *c, uint32_t sector, uint32_t *counts, size_t n_counts,
                              uint32_t *events, size_t events_capacity, uint64_t *n_events) {
+    tvs::TraceRange trace_range("tvs_compact_rings");
     if (!c) return fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null context");
     if (sector >= tvs::kSectorSteps)
         return fail(c, TVS_ERR_INVALID_ARGUMENT, "sector outside 0..180");
@@ -1016,6 +1022,7 @@ tvs_status tvs_reset_surfaces(tvs_ctx *c) {
 }
 
 tvs_status tvs_read_surfaces(tvs_ctx *c, float *out, size_t n) {
+    tvs::TraceRange trace_range("tvs_read_surfaces");
     if (!c || !out) return fail(c, TVS_ERR_INVALID_ARGUMENT, "null argument");
     if (n != c->geo.computable_points_count) return fail(c, TVS_ERR_SIZE, "surface length");
     TVS_HIP(c, hipStreamSynchronize(c->stream), "sync");

@@ -276,6 +276,7 @@ tvs_status tvs_batch_upload_f32(tvs_batch *b, uint32_t tile, const float *elevat
 }
 
 tvs_status tvs_batch_compute_bands(tvs_batch *b) {
+    tvs::TraceRange trace_range("tvs_batch_compute_bands");
     if (!b) return bfail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null batch");
     const tvs_status q = quiesce(b);
     if (q != TVS_OK) return q;
@@ -334,6 +335,7 @@ tvs_status tvs_batch_get_walk_shape(const tvs_batch *b, tvs_walk_shape *out) {
 
 tvs_status tvs_batch_autotune(tvs_batch *b, uint32_t reps, tvs_walk_shape *results,
                               size_t max_results, size_t *n_results, tvs_walk_shape *best) {
+    tvs::TraceRange trace_range("tvs_batch_autotune");
     if (!b) return bfail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null batch");
     if (reps == 0) reps = 1;
     // Only the walk launch is timed: it writes the per-sector band surfaces
@@ -421,6 +423,7 @@ tvs_status tvs_batch_set_half_heights(tvs_batch *b, uint32_t enable) {
 }
 
 tvs_status tvs_batch_run(tvs_batch *b, uint32_t use_graph) {
+    tvs::TraceRange trace_range("tvs_batch_run");
     if (!b) return bfail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null batch");
     tvs_status st = prepare(b);
     if (st != TVS_OK) return st;

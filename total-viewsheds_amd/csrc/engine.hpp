@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cstdint>
 #include <string>
@@ -11,6 +12,17 @@
 #include "../../include/tvs.h"
 
 namespace tvs {
+
+// A named range in rocprofv3's marker trace (`--marker-trace`) around the
+// host side of one entry point: upload, band data, a run's enqueue, an
+// export.  A library call and nothing else when no profiler is attached.
+class TraceRange {
+  public:
+    explicit TraceRange(const char *name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange &) = delete;
+    TraceRange &operator=(const TraceRange &) = delete;
+};
 
 // kernel.rs:13, 17, 21
 constexpr float kMaxAngle = -2000.0f;

@@ -230,6 +230,7 @@ double tvs_bt_scale(const tvs_bt_header *h) {
 // `((v - min) / range * 255).clamp(0, 255) as u8`, range 1 when flat.
 tvs_status tvs_write_heatmap_png(const char *path, const float *data, uint32_t width,
                                  uint32_t height) {
+    tvs::TraceRange trace_range("tvs_write_heatmap_png");
     if (!path || (!data && width * height)) return io_fail(nullptr, TVS_ERR_INVALID_ARGUMENT, "null argument");
     const size_t n = static_cast<size_t>(width) * height;
     float mn = INFINITY, mx = -INFINITY;
@@ -293,6 +294,7 @@ tvs_status tvs_write_heatmap_png(const char *path, const float *data, uint32_t w
 // length is not band_size - 1 values belongs to another line of sight and
 // is rejected on load rather than used (cache.rs:23-31 keeps no L).
 tvs_status tvs_band_cache_save(tvs_ctx *c, const char *state_dir, uint32_t sector) {
+    tvs::TraceRange trace_range("tvs_band_cache_save");
     if (!c || !state_dir) return io_fail(c, TVS_ERR_INVALID_ARGUMENT, "null argument");
     const size_t n = c->geo.band_deltas_size;
     std::vector<int32_t> d(n ? n : 1);
@@ -317,6 +319,7 @@ tvs_status tvs_band_cache_save(tvs_ctx *c, const char *state_dir, uint32_t secto
 }
 
 tvs_status tvs_band_cache_load(tvs_ctx *c, const char *state_dir, uint32_t sector) {
+    tvs::TraceRange trace_range("tvs_band_cache_load");
     if (!c || !state_dir) return io_fail(c, TVS_ERR_INVALID_ARGUMENT, "null argument");
     const size_t n = c->geo.band_deltas_size;
     std::vector<int32_t> d(n ? n : 1);

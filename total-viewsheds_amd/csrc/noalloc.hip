@@ -133,6 +133,7 @@ tvs_status run_chunks(tvs_ctx *c, uint32_t first, uint32_t count) {
 extern "C" {
 
 tvs_status tvs_run_sectors_noalloc(tvs_ctx *c, uint32_t first, uint32_t count) {
+    tvs::TraceRange trace_range("tvs_run_sectors_noalloc");
     if (!c) return TVS_ERR_INVALID_ARGUMENT;
     if (first >= tvs::kSectorSteps || count > tvs::kSectorSteps - first)
         return nfail(c, TVS_ERR_INVALID_ARGUMENT, "sector range outside 0..180");

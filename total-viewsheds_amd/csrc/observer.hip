@@ -141,6 +141,7 @@ extern "C" {
 tvs_status tvs_observer_walk(tvs_ctx *c, uint32_t tvs_id, uint32_t first, uint32_t count,
                              uint32_t *counts, float *band_surfaces, uint32_t *events,
                              size_t events_capacity, uint64_t *n_events) {
+    tvs::TraceRange trace_range("tvs_observer_walk");
     if (!c) return TVS_ERR_INVALID_ARGUMENT;
     if (first >= tvs::kSectorSteps || count > tvs::kSectorSteps - first)
         return ofail(c, TVS_ERR_INVALID_ARGUMENT, "sector range outside 0..180");

@@ -145,6 +145,7 @@ size_t tvs_walk_shape_count(void) { return tvs::compiled_walk_shapes().size(); }
 tvs_status tvs_autotune(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t reps,
                         tvs_walk_shape *results, size_t max_results, size_t *n_results,
                         tvs_walk_shape *best) {
+    tvs::TraceRange trace_range("tvs_autotune");
     if (!c) return TVS_ERR_INVALID_ARGUMENT;
     if (reps == 0) reps = 1;
     const uint32_t N = c->geo.computable_points_count;
