@@ -413,19 +413,30 @@ def test_leading_zero_distances_bit_exact(small, oracle, zeros):
                                       rings[1])
 
 
-def test_small_gpu_run_mirror(small, oracle):
+@pytest.mark.parametrize("heights", ["f32", "int16", "f32 fraction"])
+def test_small_gpu_run_mirror(small, oracle, heights):
     """tvs_run_sector_with_band is GPU::run (gpu.rs:387-468): one sector of
-    host band data in, cumulative surfaces out."""
-    wl, dem, deltas, dists, _ = small
-    elev = wl.elevations.astype(np.float32)
+    host band data in, cumulative surfaces out (the batched walk on the
+    scratch slot, four-operation divide).  A batched run of the context's
+    own sectors afterwards is unaffected by the scratch slot's divide."""
+    wl, dem, deltas, dists, cum = small
+    upload = {"f32": wl.elevations.astype(np.float32), "int16": wl.elevations,
+              "f32 fraction": wl.elevations.astype(np.float32) + np.float32(0.375)}[heights]
+    elev = upload.astype(np.float32)
     want = np.zeros(dem.computable_points_count, np.float32)
     with make_engine(wl) as e:
-        e.upload(elev)
-        for s in (0, 17, 91):
+        e.upload(upload)
+        for s in (0, 17, 91, 17):
             got = e.run_sector_with_band(deltas[s], dists[s])
             oracle.walk_sectors(dem, elev, wl.observer_height, deltas[s:s + 1],
                                 dists[s:s + 1], cumulative=want)
-            np.testing.assert_array_equal(got, want)
+            np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32),
+                                          err_msg=f"{heights} sector {s}")
+        if heights != "f32 fraction":
+            e.compute_bands(0, 180)
+            e.reset_surfaces()
+            e.run(0, 180, "batched")
+            np.testing.assert_array_equal(e.surfaces().view(np.uint32), cum.view(np.uint32))
 
 
 @pytest.mark.parametrize("method", ["plain", "batched"])

@@ -399,6 +399,10 @@ hipError_t launch_half_heights(hipStream_t s, const int16_t *in, uint16_t *out, 
 // (the tuner's timing runs).
 tvs_status run_sectors(tvs_ctx *c, uint32_t first, uint32_t count, tvs_method method,
                        bool keep_rings);
+// run_sectors on step-table slots [first, first + count) without the sector
+// range and band checks (the scratch slot of tvs_run_sector_with_band).
+tvs_status run_slots(tvs_ctx *c, uint32_t first, uint32_t count, tvs_method method,
+                     bool keep_rings);
 tvs_status validate_slots(tvs_ctx *c, uint32_t first, uint32_t count);
 tvs_status prepare_tile(tvs_ctx *c, uint32_t first, uint32_t count, uint32_t capture,
                         const WalkConfig &cfg, TileDesc *out, bool allocate = true);
